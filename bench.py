@@ -1,0 +1,201 @@
+"""Benchmark of the approximate-integration path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload (the single-GPU drop-in configuration): N = 2^24 equal-mass sheets
+(m = 2^-24, twopiG = 1) drawn from an isothermal sech^2 disk (z_h = 1,
+sigma = 1, RandomState(0)), external harmonic omega = 0.1, dt = 0.005,
+nleap = 20.  One bench step is one output interval (one ``next()`` of the
+generator): nleap = 20 leapfrog steps, each a sort of the positions, the
+exact force, a kick and a drift.  The state is resident in HBM before the
+timed region; the timed region is K output intervals of device work bracketed
+by stream synchronisations.  ``value`` is N * nleap * K / seconds
+(FP64 sheet-steps/s).  The end-to-end rate through the generator (with the
+per-output de-sort and device-to-host copy of x and v) is reported beside it.
+
+Roofline: the dominant kernel is timed with HIP events on the simulation's
+stream during the timed region (per-launch average); ``achieved`` is its
+algorithmic bytes per launch divided by that average, against the MI355X HBM
+peak of 8 TB/s.
+
+CPU baseline: the reference's own C (built from its sources into oracle/_ref
+by oracle/Makefile, sort='parallel', OpenMP) timed on this host on a bounded
+sample of the same workload (see ``cpu_baseline.sample``).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'FP64 sheet-steps/s @N=2^24 and batched-vs-plain; p50/p99 sim latency'
+HBM_PEAK = 8.0e12
+
+# kernel classes (include/wendy_amd.h, wdy_kernel_time)
+KC_SORT_HIST, KC_SORT_PASS, KC_LEAP, KC_SCAN, KC_OTHER, KC_WARM = range(6)
+
+
+def workload(n, seed=0):
+    """sech^2 disk, z_h = sigma = 1, equal masses 1/n (RandomState(seed))."""
+    rs = numpy.random.RandomState(seed)
+    u = rs.uniform(size=n)
+    bad = u == 0.
+    while numpy.any(bad):
+        u[bad] = rs.uniform(size=int(bad.sum()))
+        bad = u == 0.
+    x = 2. * numpy.arctanh(2. * u - 1.)
+    v = rs.normal(size=n)
+    m = numpy.full(n, 1. / n)
+    return x, v, m
+
+
+def cpu_baseline(x, v, m, dt, nleap, omega, threads, budget_s=25.):
+    """Time the reference C (sort='parallel') on a bounded sample.
+
+    One warm-up output (it contains the cold sort of the random initial
+    order), then outputs of ``k`` leapfrog steps until ``budget_s`` of CPU
+    work; rate = N * steps / time_elapsed, using the reference's own in-C
+    wall clock (wendy/wendy.c:396,416-417)."""
+    from oracle import ref_harness as R
+    if not R.available():
+        return None
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+    n = len(x)
+    h = dt / nleap
+    k = 2
+    ref = R.RefApprox(x, v, m, h * k, k, omega=omega, sort='parallel')
+    t0 = time.time()
+    ref.onestep()
+    warm = time.time() - t0
+    steps, secs = 0, 0.
+    while secs < budget_s and (time.time() - t0) < budget_s + warm + 5:
+        _, _, te = ref.onestep()
+        steps += k
+        secs += te
+    if steps == 0:
+        return None
+    return {'value': n * steps / secs, 'unit': 'sheet-steps/s',
+            'cores': threads, 'kind': 'reference',
+            'sample': ('N=%d sheets of this workload, reference C '
+                       "_wendy_nbody_approx_onestep sort='parallel' "
+                       '(OpenMP, PARALLEL_SORT_NUM_THREADS=16, '
+                       'OMP_NUM_THREADS=%d), %d leapfrog steps after one '
+                       'discarded cold-sort output, %.1f s timed by the '
+                       "reference's own wall clock" % (n, threads, steps, secs))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--n', type=int, default=1 << 24)
+    ap.add_argument('--nleap', type=int, default=20)
+    ap.add_argument('--dt', type=float, default=0.005)
+    ap.add_argument('--omega', type=float, default=0.1)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-threads', type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build runs on one GPU; --gpus %d is not '
+              'supported (multi-GPU sharding is design-only)' % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import wendy_amd
+    n, nleap, dt, omega = args.n, args.nleap, args.dt, args.omega
+    x, v, m = workload(n)
+    h = dt / nleap
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    log('bench: n=%d nleap=%d steps=%d warmup=%d' % (n, nleap, args.steps,
+                                                     args.warmup))
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), omega ** 2.)
+    for _ in range(args.warmup):
+        sim.advance(h, nleap)
+    sim.read()
+    log('bench: warmup done')
+    sim.set_timing(True)
+    sim.read()  # synchronise
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.advance(h, nleap)
+    sim.read()  # synchronise
+    t1 = time.perf_counter()
+    sim.set_timing(False)
+    elapsed = t1 - t0
+    rate = n * nleap * args.steps / elapsed
+    log('bench: device-only %.3e sheet-steps/s (%.2f ms/output)'
+        % (rate, 1e3 * elapsed / args.steps))
+    kt = {c: sim.kernel_time(c) for c in range(6)}
+
+    # end-to-end through the generator loop: advance + de-sort + D2H copy
+    xo = numpy.empty(n)
+    vo = numpy.empty(n)
+    ne = max(1, min(10, args.steps))
+    t0 = time.perf_counter()
+    for _ in range(ne):
+        sim.advance(h, nleap)
+        sim.read(xo, vo)
+    e2e = n * nleap * ne / (time.perf_counter() - t0)
+    log('bench: end-to-end %.3e sheet-steps/s' % e2e)
+    sim.close()
+
+    # dominant kernel and its algorithmic bytes per launch
+    busy = {c: kt[c][0] for c in kt}
+    dom = max(busy, key=busy.get)
+    ms, cnt = kt[dom]
+    if dom == KC_SORT_PASS:
+        name, bpl = 'radix scatter pass', 24 * n
+        note = 'read key 8 B + payload 4 B, write key 8 B + payload 4 B per sheet'
+    elif dom == KC_WARM:
+        name, bpl = 'warm-start bucket sort + leap', 40 * n
+        note = 'read key, v, idx; write key, v, idx (40 B per sheet)'
+    elif dom == KC_LEAP:
+        name, bpl = 'fused force+kick+drift', 44 * n
+        note = 'read key, perm, v, idx; write key, v, idx'
+    else:
+        name, bpl = 'class %d' % dom, 16 * n
+        note = 'approximate'
+    avg_s = ms * 1e-3 / max(cnt, 1)
+    achieved = bpl / avg_s if avg_s > 0 else 0.
+    roofline = {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9,
+                'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': None,
+                'kernel': name, 'bytes_per_launch': bpl,
+                'avg_launch_us': avg_s * 1e6, 'launches': cnt,
+                'bytes_note': note,
+                'step_frac_B236': rate * 236 / HBM_PEAK,
+                'step_frac_B40': rate * 40 / HBM_PEAK}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log('bench: cpu baseline ...')
+        cpu = cpu_baseline(x, v, m, dt, nleap, omega, args.cpu_threads)
+        if cpu:
+            log('bench: cpu baseline %.3e sheet-steps/s' % cpu['value'])
+
+    out = {
+        'metric': METRIC, 'value': rate, 'unit': 'sheet-steps/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': ('plain single-simulation drop-in: N=2^24 '
+                                'equal-mass sech^2 disk (z_h=1, sigma=1), '
+                                'omega=0.1, dt=0.005, nleap=20; one step = '
+                                'one output interval'),
+                   'n': n, 'nleap': nleap, 'dt': dt, 'omega': omega},
+        'e2e_value': e2e,
+        'kernel_ms': {str(c): kt[c][0] for c in kt},
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

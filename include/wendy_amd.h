@@ -1,0 +1,158 @@
+/*
+ * wendy_amd.h -- C ABI of libwendy_amd.so, the MI355X (gfx950) implementation
+ * of wendy's approximate-integration path (drift-kick-drift leapfrog with
+ * exact 1-D self-gravity).
+ *
+ * Plain C types only: int64_t sizes, double/int arrays in host memory,
+ * opaque handles.  Every entry point returns 0 (WDY_OK) on success or a
+ * negative WDY_E* code, leaves a message for wdy_last_error(), never throws
+ * across the ABI and never exits the process.  A wdy_sim owns its device
+ * buffers and one HIP stream; calls on one handle must be serialised by the
+ * caller, different handles may be driven from different threads.
+ *
+ * Reference counterparts (paths relative to the reference repository):
+ *   the whole sim-handle API replaces the per-call state hand-over of
+ *   _wendy_nbody_approx_onestep (wendy/wendy.c:385-418, prototype
+ *   wendy/wendy.h:29-34), which wendy/wendy.py:76-93 binds with ctypes and
+ *   wendy/wendy.py:424-433 calls once per output.
+ */
+#ifndef WENDY_AMD_H
+#define WENDY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WDY_ABI_VERSION 1
+
+/* return codes */
+#define WDY_OK 0
+#define WDY_EINVAL (-1)   /* invalid argument */
+#define WDY_ENOMEM (-2)   /* device allocation failed */
+#define WDY_ERUNTIME (-3) /* HIP runtime / launch failure */
+#define WDY_EVERIFY (-4)  /* on-device verification failed */
+#define WDY_ENODEV (-5)   /* no usable GPU */
+#define WDY_EUNSUPPORTED (-6)
+
+/* wdy_create flags */
+#define WDY_EQUAL_MASS 1  /* every m[i] is the same value: force from rank */
+/* Path selection.  By default n up to the per-workgroup LDS capacity
+ * (12 B per sheet) runs as one persistent workgroup per output interval
+ * ("small" path), larger n as a multi-kernel step over the whole GPU
+ * ("large" path).  These flags force one path, e.g. so that tests reach the
+ * large path's code at small n. */
+#define WDY_FORCE_LARGE 2
+#define WDY_FORCE_SMALL 4
+
+typedef struct wdy_sim wdy_sim;
+
+/* Version of this ABI (WDY_ABI_VERSION).  No reference counterpart. */
+int wdy_abi_version(void);
+
+/* Message for the last nonzero return on this thread.  Replaces the
+ * reference's warnings.warn/RuntimeError text (wendy/wendy.py:30-35,195-202). */
+const char *wdy_last_error(void);
+
+/* Number of visible GPUs (0 and WDY_ENODEV when there is none). */
+int wdy_device_count(int *count);
+
+/* Create a simulation on the current HIP device and upload its state.
+ *   x, v, m   host arrays of n doubles; m is already multiplied by twopiG,
+ *             exactly as wendy/wendy.py:371 does before its C call
+ *   totmass   numpy.sum(m) computed by the host (wendy/wendy.py:383)
+ *   omega2    omega**2, or any negative value for no harmonic term
+ *             (the -1 sentinel of wendy/wendy.py:363-366)
+ *   flags     WDY_EQUAL_MASS when all m are equal
+ * The caller keeps ownership of the host arrays.  Replaces the setup of
+ * wendy/wendy.py:363-387 (xi array, copies, cumulmass). */
+int wdy_create(int64_t n, const double *x, const double *v, const double *m,
+               double totmass, double omega2, int flags, wdy_sim **out);
+
+/* Enqueue one output interval: half drift, nleap x (sort, force, kick,
+ * drift), last drift halved -- the body of _wendy_nbody_approx_onestep
+ * (wendy/wendy.c:394-411) with h = dt/nleap.  Asynchronous. */
+int wdy_advance(wdy_sim *sim, double h, int nleap);
+
+/* De-sort the state into original sheet order and copy it to the host:
+ * x_out[j], v_out[j] for sheet j (wendy/wendy.c:413-415).  Blocks until the
+ * copy is done.  seconds_out (may be NULL) receives the device time of the
+ * last wdy_advance, the counterpart of time_elapsed (wendy/wendy.c:396,416).
+ * Either array pointer may be NULL. */
+int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out);
+
+/* Total energy of the current device state, computed on the device without
+ * copying the state out: twopiG * sum_{i<j} m_i m_j |x_i - x_j| +
+ * sum m v^2/2 (+ sum m omega^2 x^2/2 when omega2 >= 0), with m the masses
+ * given to wdy_create divided by twopiG.  Counterpart of wendy.energy
+ * (wendy/wendy.py:439-475) evaluated on the generator's live state. */
+int wdy_energy(wdy_sim *sim, double twopiG, double omega2, double *e_out);
+
+/* Free the simulation's device buffers and stream. */
+int wdy_destroy(wdy_sim *sim);
+
+/* Per-kernel-class device time, for benchmarks.  wdy_set_timing(sim, 1)
+ * brackets every subsequent launch with HIP events on the sim's stream;
+ * wdy_kernel_time returns the summed milliseconds and launch count of one
+ * class since timing was enabled (classes: 0 radix histogram, 1 radix
+ * scatter pass, 2 fused leap, 3 mass prefix, 4 other, 5 warm-start sort).
+ * No reference counterpart. */
+int wdy_set_timing(wdy_sim *sim, int on);
+int wdy_kernel_time(wdy_sim *sim, int kernel_class, double *ms_total,
+                    int64_t *count);
+
+/* ---- stateless diagnostics on host arrays (uploaded, computed on the GPU) */
+
+/* wendy.energy(x, v, m, twopiG, individual=False, omega)
+ * (wendy/wendy.py:439-475); omega2 < 0: no harmonic term. */
+int wdy_energy_arrays(int64_t n, const double *x, const double *v,
+                      const double *m, double twopiG, double omega2,
+                      double *e_out);
+
+/* wendy.energy(..., individual=True) (wendy/wendy.py:458-466): out[n]. */
+int wdy_energy_individual_arrays(int64_t n, const double *x, const double *v,
+                                 const double *m, double twopiG, double omega2,
+                                 double *out);
+
+/* wendy.momentum(v, m) (wendy/wendy.py:477-491). */
+int wdy_momentum_arrays(int64_t n, const double *v, const double *m,
+                        double *p_out);
+
+/* wendy.potential(y, x, v, m, twopiG, omega) (wendy/wendy.py:493-517):
+ * out[ny]. */
+int wdy_potential_arrays(int64_t ny, const double *y, int64_t n,
+                         const double *x, const double *m, double twopiG,
+                         double omega2, double *out);
+
+/* ---- reference-signature compatibility entry point ----------------------
+ * Same name, arguments and meaning as the reference's
+ * _wendy_nbody_approx_onestep (wendy/wendy.c:385-393, wendy/wendy.h:29-34),
+ * so the reference's own ctypes binding (wendy/wendy.py:76-93) can load this
+ * library instead of its C extension.  Each call uploads xi/v/m, runs the
+ * nleap leapfrog steps on the GPU, and writes back xi (sorted), x
+ * (de-sorted), v, a (last force, original order) and cumulmass (last mass
+ * prefix, sorted order) and *time_elapsed (device seconds).  *t0 is not
+ * touched (it only advances with ext_force).  Unlike the reference, which
+ * ignores both, a non-NULL ext_force sets *err = WDY_EUNSUPPORTED and an
+ * unknown sort_type (not 0..4) sets *err = WDY_EINVAL, without computing;
+ * every sort_type maps to the device sort.  Other failures set *err to the
+ * WDY_E* code. */
+struct array_w_index {
+  int idx;
+  double val;
+};
+void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
+                                 double *v, double *m, double *a,
+                                 double totmass, double dt, int nleap,
+                                 double *t0, double omega2,
+                                 double (*ext_force)(int, double *, double,
+                                                     double *),
+                                 int sort_type, int *err,
+                                 double *time_elapsed, double *cumulmass);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WENDY_AMD_H */

@@ -1,0 +1,132 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every symbol include/wendy_amd.h declares, and the Python API reproduces the
+reference's argument handling (lazy errors, error types, messages)."""
+import ctypes
+import os
+import re
+
+import numpy
+import pytest
+
+import wendy_amd
+from wendy_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'wendy_amd.h')
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    names = re.findall(r'\b(\w+)\s*\([^;{]*\)\s*;', text)
+    return sorted(set(n for n in names
+                      if n.startswith('wdy_') or n.startswith('_wendy')))
+
+
+def have_gpu():
+    n = ctypes.c_int(0)
+    return _lib.lib().wdy_device_count(ctypes.byref(n)) == 0 and n.value > 0
+
+
+def test_library_built_and_loads():
+    assert os.path.exists(_lib.LIB_PATH), 'run python -m wendy_amd.build'
+    lib = _lib.lib()
+    assert lib.wdy_abi_version() == 1
+
+
+def test_every_declared_symbol_is_exported():
+    declared = header_functions()
+    assert '_wendy_nbody_approx_onestep' in declared
+    assert set(declared) == set(_lib.EXPORTED)
+    handle = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(handle, name), name
+
+
+def test_no_device_reports_enodev():
+    if have_gpu():
+        pytest.skip('a GPU is visible')
+    lib = _lib.lib()
+    x = numpy.zeros(4)
+    h = ctypes.c_void_p()
+    rc = lib.wdy_create(4, x, x, x + 1., 4., -1., 1, ctypes.byref(h))
+    assert rc == _lib.WDY_ENODEV
+    assert b'GPU' in lib.wdy_last_error()
+    with pytest.raises(_lib.WendyError):
+        wendy_amd.Sim(x, x, x + 1., 4., -1.)
+
+
+def test_create_rejects_bad_arguments_before_touching_device():
+    lib = _lib.lib()
+    x = numpy.array([0., 1., numpy.nan])
+    h = ctypes.c_void_p()
+    assert lib.wdy_create(3, x, x, x, 1., -1., 0, ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert b'non-finite' in lib.wdy_last_error()
+    y = numpy.array([0., 1., 2.])
+    assert lib.wdy_create(0, y, y, y, 1., -1., 0, ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert lib.wdy_create(3, y, y, y, 1., -1., 8, ctypes.byref(h)) == _lib.WDY_EINVAL
+    # WDY_EQUAL_MASS with unequal masses is refused
+    assert lib.wdy_create(3, y, y, y, 1., -1., 1, ctypes.byref(h)) == _lib.WDY_EINVAL
+    with pytest.raises(ValueError):
+        _lib.check(_lib.WDY_EINVAL)
+
+
+def test_nleap_error_is_lazy_with_reference_message():
+    """tests/test_approx.py:187-196 of the reference."""
+    x = numpy.array([-1., 1.])
+    v = numpy.array([0., 0.])
+    m = numpy.array([1., 1.])
+    g = wendy_amd.nbody(x, v, m, 2, approx=True)
+    with pytest.raises(ValueError) as excinfo:
+        next(g)
+    assert str(excinfo.value) == ('When approx is True, the number of leapfrog '
+                                  'steps nleap= per output time step needs to '
+                                  'be set')
+
+
+def test_unknown_sort_raises_keyerror_lazily():
+    g = wendy_amd.nbody(numpy.zeros(2), numpy.zeros(2), numpy.ones(2), 0.1,
+                        approx=True, nleap=2, sort='bogus')
+    with pytest.raises(KeyError):
+        next(g)
+
+
+def test_exact_solver_and_ext_force_not_provided():
+    g = wendy_amd.nbody(numpy.zeros(2), numpy.zeros(2), numpy.ones(2), 0.1)
+    with pytest.raises(NotImplementedError):
+        next(g)
+    g = wendy_amd.nbody(numpy.zeros(2), numpy.zeros(2), numpy.ones(2), 0.1,
+                        approx=True, nleap=2, ext_force=lambda x, t: 0.)
+    with pytest.raises(NotImplementedError):
+        next(g)
+
+
+def test_compat_symbol_rejects_ext_force_and_bad_sort():
+    """The reference-signature entry point refuses what it cannot do, without
+    touching the device."""
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    f = lib._wendy_nbody_approx_onestep
+    f.restype = None
+    xi = numpy.zeros(2, dtype=[('idx', '<i4'), ('pad', '<i4'), ('val', '<f8')])
+    z = numpy.zeros(2)
+    err = ctypes.c_int(0)
+    te = ctypes.c_double(0)
+    t0 = ctypes.c_double(0)
+    cb = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int,
+                          ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                          ctypes.POINTER(ctypes.c_double))(lambda *a: 0.)
+    args = lambda ext, st: (ctypes.c_int(2), xi.ctypes.data_as(ctypes.c_void_p),
+                            z.ctypes.data_as(ctypes.c_void_p),
+                            z.ctypes.data_as(ctypes.c_void_p),
+                            z.ctypes.data_as(ctypes.c_void_p),
+                            z.ctypes.data_as(ctypes.c_void_p),
+                            ctypes.c_double(1.), ctypes.c_double(0.1),
+                            ctypes.c_int(2), ctypes.byref(t0),
+                            ctypes.c_double(-1.), ext, ctypes.c_int(st),
+                            ctypes.byref(err), ctypes.byref(te),
+                            z.ctypes.data_as(ctypes.c_void_p))
+    f(*args(cb, 0))
+    assert err.value == _lib.WDY_EUNSUPPORTED
+    err.value = 0
+    f(*args(None, 7))
+    assert err.value == _lib.WDY_EINVAL

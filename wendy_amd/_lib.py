@@ -1,0 +1,123 @@
+"""ctypes binding of libwendy_amd.so (the C ABI in include/wendy_amd.h).
+
+The library is loaded by absolute path from this package directory, the way
+the reference loads its C extension with ``ctypes.CDLL``
+(wendy/wendy.py:13-37).  Unlike the reference, which only warns when its
+extension is missing and then fails on first use, a missing library raises
+ImportError-like ``RuntimeError`` immediately on first use: there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy
+from numpy.ctypeslib import ndpointer
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libwendy_amd.so')
+
+WDY_OK = 0
+WDY_EINVAL = -1
+WDY_ENOMEM = -2
+WDY_ERUNTIME = -3
+WDY_EVERIFY = -4
+WDY_ENODEV = -5
+WDY_EUNSUPPORTED = -6
+WDY_EQUAL_MASS = 1
+WDY_FORCE_LARGE = 2
+WDY_FORCE_SMALL = 4
+
+# every C symbol include/wendy_amd.h declares
+EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
+            'wdy_create', 'wdy_advance', 'wdy_read', 'wdy_energy',
+            'wdy_destroy', 'wdy_set_timing', 'wdy_kernel_time',
+            'wdy_energy_arrays', 'wdy_energy_individual_arrays',
+            'wdy_momentum_arrays', 'wdy_potential_arrays',
+            '_wendy_nbody_approx_onestep')
+
+_F64 = ndpointer(dtype=numpy.float64, flags=('C_CONTIGUOUS',))
+_F64W = ndpointer(dtype=numpy.float64, flags=('C_CONTIGUOUS', 'WRITEABLE'))
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+
+
+class WendyError(RuntimeError):
+    """A nonzero return code from libwendy_amd.so."""
+
+    def __init__(self, code, msg):
+        super().__init__('%s (code %d)' % (msg, code))
+        self.code = code
+
+
+def _declare(lib):
+    S = ctypes.c_void_p
+    lib.wdy_abi_version.restype = ctypes.c_int
+    lib.wdy_abi_version.argtypes = []
+    lib.wdy_last_error.restype = ctypes.c_char_p
+    lib.wdy_last_error.argtypes = []
+    lib.wdy_device_count.restype = ctypes.c_int
+    lib.wdy_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.wdy_create.restype = ctypes.c_int
+    lib.wdy_create.argtypes = [ctypes.c_int64, _F64, _F64, _F64,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                               ctypes.POINTER(S)]
+    lib.wdy_advance.restype = ctypes.c_int
+    lib.wdy_advance.argtypes = [S, ctypes.c_double, ctypes.c_int]
+    lib.wdy_read.restype = ctypes.c_int
+    lib.wdy_read.argtypes = [S, S, S, _c_double_p]
+    lib.wdy_energy.restype = ctypes.c_int
+    lib.wdy_energy.argtypes = [S, ctypes.c_double, ctypes.c_double, _c_double_p]
+    lib.wdy_destroy.restype = ctypes.c_int
+    lib.wdy_destroy.argtypes = [S]
+    lib.wdy_set_timing.restype = ctypes.c_int
+    lib.wdy_set_timing.argtypes = [S, ctypes.c_int]
+    lib.wdy_kernel_time.restype = ctypes.c_int
+    lib.wdy_kernel_time.argtypes = [S, ctypes.c_int, _c_double_p, _c_int64_p]
+    lib.wdy_energy_arrays.restype = ctypes.c_int
+    lib.wdy_energy_arrays.argtypes = [ctypes.c_int64, _F64, _F64, _F64,
+                                      ctypes.c_double, ctypes.c_double,
+                                      _c_double_p]
+    lib.wdy_energy_individual_arrays.restype = ctypes.c_int
+    lib.wdy_energy_individual_arrays.argtypes = [
+        ctypes.c_int64, _F64, _F64, _F64, ctypes.c_double, ctypes.c_double,
+        _F64W]
+    lib.wdy_momentum_arrays.restype = ctypes.c_int
+    lib.wdy_momentum_arrays.argtypes = [ctypes.c_int64, _F64, _F64,
+                                        _c_double_p]
+    lib.wdy_potential_arrays.restype = ctypes.c_int
+    lib.wdy_potential_arrays.argtypes = [ctypes.c_int64, _F64, ctypes.c_int64,
+                                         _F64, _F64, ctypes.c_double,
+                                         ctypes.c_double, _F64W]
+
+
+def lib():
+    """The loaded library; raises RuntimeError if it was never built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                'libwendy_amd.so not found at %s: build it with '
+                '`python -m wendy_amd.build` (there is no CPU fallback)'
+                % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        _declare(handle)
+        _lib = handle
+    return _lib
+
+
+def check(rc):
+    """Map a WDY_E* return code to a Python exception."""
+    if rc == WDY_OK:
+        return
+    msg = lib().wdy_last_error().decode(errors='replace')
+    if rc == WDY_EINVAL:
+        raise ValueError(msg)
+    if rc == WDY_ENOMEM:
+        raise MemoryError(msg)
+    raise WendyError(rc, msg)
+
+
+def f64(a):
+    return numpy.ascontiguousarray(a, dtype=numpy.float64)

@@ -1,0 +1,495 @@
+// C ABI of libwendy_amd.so (declared in include/wendy_amd.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "wdy_internal.h"
+
+namespace wdy {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+// ---- kernel timing ----------------------------------------------------------
+struct Timer {
+  struct Pair {
+    hipEvent_t a, b;
+  };
+  std::vector<Pair> open[KC_COUNT];  // recorded, not yet folded
+  std::vector<hipEvent_t> pool;
+  double ms[KC_COUNT] = {};
+  int64_t count[KC_COUNT] = {};
+  hipEvent_t pending[KC_COUNT] = {};
+};
+
+static hipEvent_t ev_get(Timer *t) {
+  if (!t->pool.empty()) {
+    hipEvent_t e = t->pool.back();
+    t->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void fold(Timer *t, int kc) {
+  for (auto &p : t->open[kc]) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess &&
+        hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      t->ms[kc] += ms;
+      t->count[kc] += 1;
+    }
+    t->pool.push_back(p.a);
+    t->pool.push_back(p.b);
+  }
+  t->open[kc].clear();
+}
+
+Timer *timer_create() { return new (std::nothrow) Timer(); }
+
+void timer_destroy(Timer *t) {
+  if (!t) return;
+  for (int kc = 0; kc < KC_COUNT; ++kc) {
+    for (auto &p : t->open[kc]) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    if (t->pending[kc]) (void)hipEventDestroy(t->pending[kc]);
+  }
+  for (auto e : t->pool) (void)hipEventDestroy(e);
+  delete t;
+}
+
+void timer_begin(Timer *t, hipStream_t s, int kc) {
+  hipEvent_t e = ev_get(t);
+  if (!e) return;
+  (void)hipEventRecord(e, s);
+  t->pending[kc] = e;
+}
+
+void timer_end(Timer *t, hipStream_t s, int kc) {
+  hipEvent_t a = t->pending[kc];
+  if (!a) return;
+  t->pending[kc] = nullptr;
+  hipEvent_t b = ev_get(t);
+  if (!b) {
+    t->pool.push_back(a);
+    return;
+  }
+  (void)hipEventRecord(b, s);
+  t->open[kc].push_back({a, b});
+  if (t->open[kc].size() >= 4096) fold(t, kc);
+}
+
+int timer_collect(Timer *t, int kc, double *ms_total, int64_t *count) {
+  fold(t, kc);
+  *ms_total = t->ms[kc];
+  *count = t->count[kc];
+  return WDY_OK;
+}
+
+// ---- sim lifecycle ----------------------------------------------------------
+__global__ void k_iota(u32 n, i32 *__restrict__ idx) {
+  const u32 i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) idx[i] = (i32)i;
+}
+
+static void free_sim(wdy_sim *s) {
+  if (!s) return;
+  for (int b = 0; b < 2; ++b) {
+    if (s->key[b]) (void)hipFree(s->key[b]);
+    if (s->vel[b]) (void)hipFree(s->vel[b]);
+    if (s->idx[b]) (void)hipFree(s->idx[b]);
+    if (s->mass[b]) (void)hipFree(s->mass[b]);
+    if (s->perm[b]) (void)hipFree(s->perm[b]);
+  }
+  if (s->skey) (void)hipFree(s->skey);
+  if (s->cum) (void)hipFree(s->cum);
+  if (s->tile_sums) (void)hipFree(s->tile_sums);
+  if (s->red) (void)hipFree(s->red);
+  if (s->out_x) (void)hipFree(s->out_x);
+  if (s->out_v) (void)hipFree(s->out_v);
+  radix_free(&s->radix);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->timer) timer_destroy(s->timer);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// Creates a sim from positions given per slot (x_slot[i] is the position of
+// sheet id_slot[i]; id_slot null means the identity) and velocities/masses
+// given per sheet (v_id[j], m_id[j]).  The positions are sorted on the device
+// at once, so the state is sorted before the first step.
+static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
+                       const double *v_id, const double *m_id, double totmass,
+                       double omega2, int flags, wdy_sim **out) {
+  if (!out) return fail(WDY_EINVAL, "wdy_create: out is NULL");
+  *out = nullptr;
+  if (n < 1 || n >= (int64_t(1) << 30))
+    return fail(WDY_EINVAL, "wdy_create: n=%lld outside [1, 2^30)", (long long)n);
+  if (!x_slot || !v_id || !m_id) return fail(WDY_EINVAL, "wdy_create: NULL array");
+  if (flags & ~(WDY_EQUAL_MASS | WDY_FORCE_LARGE | WDY_FORCE_SMALL))
+    return fail(WDY_EINVAL, "wdy_create: unknown flags 0x%x", flags);
+  if ((flags & WDY_FORCE_LARGE) && (flags & WDY_FORCE_SMALL))
+    return fail(WDY_EINVAL, "wdy_create: WDY_FORCE_LARGE and WDY_FORCE_SMALL");
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(x_slot[i]) || !std::isfinite(v_id[i]) ||
+        !std::isfinite(m_id[i]))
+      return fail(WDY_EINVAL, "wdy_create: non-finite input at sheet %lld",
+                  (long long)i);
+  const bool eq = flags & WDY_EQUAL_MASS;
+  if (eq)
+    for (int64_t i = 1; i < n; ++i)
+      if (m_id[i] != m_id[0])
+        return fail(WDY_EINVAL,
+                    "wdy_create: WDY_EQUAL_MASS set but m[%lld] != m[0]",
+                    (long long)i);
+  if (!std::isfinite(totmass) || !std::isfinite(omega2))
+    return fail(WDY_EINVAL, "wdy_create: non-finite totmass or omega2");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(WDY_ENODEV, "wdy_create: no GPU visible");
+
+  wdy_sim *s = new (std::nothrow) wdy_sim();
+  if (!s) return fail(WDY_ENOMEM, "wdy_create: host allocation failed");
+  int rc = [&]() -> int {
+    WDY_HIP(hipGetDevice(&s->device));
+    const u32 smax = small_max_n(s->device);
+    if ((flags & WDY_FORCE_SMALL) && (u64)n > smax)
+      return fail(WDY_EINVAL, "wdy_create: WDY_FORCE_SMALL but n=%lld > %u",
+                  (long long)n, smax);
+    s->small = (flags & WDY_FORCE_SMALL) ||
+               (!(flags & WDY_FORCE_LARGE) && (u64)n <= smax);
+    WDY_HIP(hipStreamCreate(&s->stream));
+    WDY_HIP(hipEventCreate(&s->ev0));
+    WDY_HIP(hipEventCreate(&s->ev1));
+    s->timer = timer_create();
+    if (!s->timer) return fail(WDY_ENOMEM, "wdy_create: timer allocation failed");
+    s->n = n;
+    s->equal_mass = eq;
+    s->m_equal = m_id[0];
+    s->totmass = totmass;
+    s->omega2 = omega2 < 0.0 ? -1.0 : omega2;
+    const size_t N = (size_t)n;
+    for (int b = 0; b < 2; ++b) {
+      WDY_HIP(hipMalloc(&s->key[b], N * sizeof(u64)));
+      WDY_HIP(hipMalloc(&s->idx[b], N * sizeof(i32)));
+      WDY_HIP(hipMalloc(&s->perm[b], N * sizeof(u32)));
+      if (b == 0 || !s->small) {
+        WDY_HIP(hipMalloc(&s->vel[b], N * sizeof(double)));
+        if (!eq) WDY_HIP(hipMalloc(&s->mass[b], N * sizeof(double)));
+      }
+    }
+    WDY_HIP(hipMalloc(&s->skey, N * sizeof(u64)));
+    WDY_HIP(hipMalloc(&s->cum, N * sizeof(double)));
+    WDY_HIP(hipMalloc(&s->tile_sums, (N / 4096 + 2) * sizeof(double)));
+    WDY_HIP(hipMalloc(&s->red, (N / 256 + 4) * sizeof(double)));
+    WDY_HIP(hipMalloc(&s->out_x, N * sizeof(double)));
+    WDY_HIP(hipMalloc(&s->out_v, N * sizeof(double)));
+    int r = radix_alloc(&s->radix, (u32)n);
+    if (r) return r;
+    // staging: x -> out_x, v -> out_v, m -> cum, sheet ids -> idx[1]
+    WDY_HIP(hipMemcpyAsync(s->out_x, x_slot, N * 8, hipMemcpyHostToDevice,
+                           s->stream));
+    WDY_HIP(hipMemcpyAsync(s->out_v, v_id, N * 8, hipMemcpyHostToDevice,
+                           s->stream));
+    if (!eq)
+      WDY_HIP(hipMemcpyAsync(s->cum, m_id, N * 8, hipMemcpyHostToDevice,
+                             s->stream));
+    if (id_slot) {
+      WDY_HIP(hipMemcpyAsync(s->idx[1], id_slot, N * 4, hipMemcpyHostToDevice,
+                             s->stream));
+    } else {
+      k_iota<<<(unsigned)((N + 255) / 256), 256, 0, s->stream>>>((u32)n,
+                                                                  s->idx[1]);
+      WDY_CHECK_LAUNCH();
+    }
+    r = launch_build_state(s, s->out_x, s->idx[1], s->out_v, s->cum);
+    if (r) return r;
+    WDY_HIP(hipStreamSynchronize(s->stream));
+    return WDY_OK;
+  }();
+  if (rc) {
+    free_sim(s);
+    return rc;
+  }
+  *out = s;
+  return WDY_OK;
+}
+
+static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
+                        double *cum_last) {
+  WDY_HIP(hipSetDevice(s->device));
+  WDY_HIP(hipEventRecord(s->ev0, s->stream));
+  if (s->small) {
+    int rc = launch_small_advance(s, h, nleap, acc_last, cum_last);
+    if (rc) return rc;
+  } else {
+    int rc = launch_drift(s, h / 2.);
+    if (rc) return rc;
+    for (int ii = 0; ii < nleap - 1; ++ii) {
+      rc = launch_leap(s, h, h, nullptr, nullptr);
+      if (rc) return rc;
+    }
+    rc = launch_leap(s, h, h / 2., acc_last, cum_last);
+    if (rc) return rc;
+  }
+  WDY_HIP(hipEventRecord(s->ev1, s->stream));
+  return WDY_OK;
+}
+
+static int check_sim(wdy_sim *s, const char *fn) {
+  if (!s) return fail(WDY_EINVAL, "%s: sim is NULL", fn);
+  return WDY_OK;
+}
+
+}  // namespace wdy
+
+using namespace wdy;
+
+extern "C" {
+
+int wdy_abi_version(void) { return WDY_ABI_VERSION; }
+
+const char *wdy_last_error(void) { return g_err; }
+
+int wdy_device_count(int *count) {
+  if (!count) return fail(WDY_EINVAL, "wdy_device_count: count is NULL");
+  *count = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return fail(WDY_ENODEV, "no GPU visible");
+  *count = n;
+  return WDY_OK;
+}
+
+int wdy_create(int64_t n, const double *x, const double *v, const double *m,
+               double totmass, double omega2, int flags, wdy_sim **out) {
+  return create_impl(n, x, nullptr, v, m, totmass, omega2, flags, out);
+}
+
+int wdy_advance(wdy_sim *sim, double h, int nleap) {
+  if (int rc = check_sim(sim, "wdy_advance")) return rc;
+  if (nleap < 1) return fail(WDY_EINVAL, "wdy_advance: nleap=%d < 1", nleap);
+  if (!std::isfinite(h)) return fail(WDY_EINVAL, "wdy_advance: h not finite");
+  return advance_impl(sim, h, nleap, nullptr, nullptr);
+}
+
+int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out) {
+  if (int rc = check_sim(sim, "wdy_read")) return rc;
+  WDY_HIP(hipSetDevice(sim->device));
+  const size_t N = (size_t)sim->n;
+  if (x_out || v_out) {
+    if (int rc = launch_desort(sim, sim->out_x, sim->out_v, nullptr, nullptr,
+                               nullptr))
+      return rc;
+    if (x_out)
+      WDY_HIP(hipMemcpyAsync(x_out, sim->out_x, N * 8, hipMemcpyDeviceToHost,
+                             sim->stream));
+    if (v_out)
+      WDY_HIP(hipMemcpyAsync(v_out, sim->small ? sim->vel[0] : sim->out_v,
+                             N * 8, hipMemcpyDeviceToHost, sim->stream));
+  }
+  WDY_HIP(hipStreamSynchronize(sim->stream));
+  if (seconds_out) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, sim->ev0, sim->ev1) != hipSuccess) ms = 0.f;
+    *seconds_out = ms * 1e-3;
+  }
+  return WDY_OK;
+}
+
+int wdy_energy(wdy_sim *sim, double twopiG, double omega2, double *e_out) {
+  if (int rc = check_sim(sim, "wdy_energy")) return rc;
+  if (!e_out) return fail(WDY_EINVAL, "wdy_energy: e_out is NULL");
+  if (!(twopiG != 0.0) || !std::isfinite(twopiG))
+    return fail(WDY_EINVAL, "wdy_energy: twopiG must be finite and nonzero");
+  WDY_HIP(hipSetDevice(sim->device));
+  return launch_energy(sim, twopiG, omega2, e_out);
+}
+
+int wdy_destroy(wdy_sim *sim) {
+  if (!sim) return WDY_OK;
+  (void)hipSetDevice(sim->device);
+  (void)hipStreamSynchronize(sim->stream);
+  free_sim(sim);
+  return WDY_OK;
+}
+
+int wdy_set_timing(wdy_sim *sim, int on) {
+  if (int rc = check_sim(sim, "wdy_set_timing")) return rc;
+  sim->timing = on != 0;
+  return WDY_OK;
+}
+
+int wdy_kernel_time(wdy_sim *sim, int kernel_class, double *ms_total,
+                    int64_t *count) {
+  if (int rc = check_sim(sim, "wdy_kernel_time")) return rc;
+  if (kernel_class < 0 || kernel_class >= KC_COUNT || !ms_total || !count)
+    return fail(WDY_EINVAL, "wdy_kernel_time: bad arguments");
+  WDY_HIP(hipSetDevice(sim->device));
+  return timer_collect(sim->timer, kernel_class, ms_total, count);
+}
+
+static int check_diag(int64_t n, const char *fn) {
+  if (n < 0 || n >= (int64_t(1) << 30))
+    return fail(WDY_EINVAL, "%s: n=%lld outside [0, 2^30)", fn, (long long)n);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(WDY_ENODEV, "%s: no GPU visible", fn);
+  return WDY_OK;
+}
+
+int wdy_energy_arrays(int64_t n, const double *x, const double *v,
+                      const double *m, double twopiG, double omega2,
+                      double *e_out) {
+  if (int rc = check_diag(n, "wdy_energy_arrays")) return rc;
+  if (!e_out || (n > 0 && (!x || !v || !m)))
+    return fail(WDY_EINVAL, "wdy_energy_arrays: NULL pointer");
+  if (n == 0) {
+    *e_out = 0.0;
+    return WDY_OK;
+  }
+  return launch_energy_arrays(n, x, v, m, twopiG, omega2, e_out);
+}
+
+int wdy_energy_individual_arrays(int64_t n, const double *x, const double *v,
+                                 const double *m, double twopiG, double omega2,
+                                 double *out) {
+  if (int rc = check_diag(n, "wdy_energy_individual_arrays")) return rc;
+  if (n == 0) return WDY_OK;
+  if (!x || !v || !m || !out)
+    return fail(WDY_EINVAL, "wdy_energy_individual_arrays: NULL pointer");
+  return launch_energy_individual_arrays(n, x, v, m, twopiG, omega2, out);
+}
+
+int wdy_momentum_arrays(int64_t n, const double *v, const double *m,
+                        double *p_out) {
+  if (int rc = check_diag(n, "wdy_momentum_arrays")) return rc;
+  if (!p_out || (n > 0 && (!v || !m)))
+    return fail(WDY_EINVAL, "wdy_momentum_arrays: NULL pointer");
+  if (n == 0) {
+    *p_out = 0.0;
+    return WDY_OK;
+  }
+  return launch_momentum_arrays(n, v, m, p_out);
+}
+
+int wdy_potential_arrays(int64_t ny, const double *y, int64_t n,
+                         const double *x, const double *m, double twopiG,
+                         double omega2, double *out) {
+  if (int rc = check_diag(n, "wdy_potential_arrays")) return rc;
+  if (ny < 0 || ny >= (int64_t(1) << 31))
+    return fail(WDY_EINVAL, "wdy_potential_arrays: bad ny");
+  if (ny == 0) return WDY_OK;
+  if (!y || !out || (n > 0 && (!x || !m)))
+    return fail(WDY_EINVAL, "wdy_potential_arrays: NULL pointer");
+  return launch_potential_arrays(ny, y, n, x, m, twopiG, omega2, out);
+}
+
+void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
+                                 double *v, double *m, double *a,
+                                 double totmass, double dt, int nleap,
+                                 double *t0, double omega2,
+                                 double (*ext_force)(int, double *, double,
+                                                     double *),
+                                 int sort_type, int *err,
+                                 double *time_elapsed, double *cumulmass) {
+  (void)t0;
+  auto set_err = [&](int code) {
+    if (err) *err = code;
+  };
+  if (ext_force) {
+    fail(WDY_EUNSUPPORTED, "ext_force is not supported on the GPU path");
+    return set_err(WDY_EUNSUPPORTED);
+  }
+  if (sort_type < 0 || sort_type > 4) {
+    fail(WDY_EINVAL, "unknown sort_type %d", sort_type);
+    return set_err(WDY_EINVAL);
+  }
+  if (N < 1 || !xi || !x || !v || !m || nleap < 1) {
+    fail(WDY_EINVAL, "invalid arguments");
+    return set_err(WDY_EINVAL);
+  }
+  const size_t n = (size_t)N;
+  std::vector<double> xs(n);
+  std::vector<i32> ids(n);
+  std::vector<char> seen(n, 0);
+  bool eq = true;
+  for (size_t i = 0; i < n; ++i) {
+    const int j = xi[i].idx;
+    if (j < 0 || (size_t)j >= n || seen[j]) {
+      fail(WDY_EINVAL, "xi[].idx is not a permutation of 0..N-1 (at %zu)", i);
+      return set_err(WDY_EINVAL);
+    }
+    seen[j] = 1;
+    ids[i] = j;
+    xs[i] = xi[i].val;
+    eq = eq && (m[i] == m[0]);
+  }
+  wdy_sim *s = nullptr;
+  int rc = create_impl(N, xs.data(), ids.data(), v, m, totmass, omega2,
+                       eq ? WDY_EQUAL_MASS : 0, &s);
+  if (rc) return set_err(rc);
+  double *acc = nullptr, *acc_o = nullptr, *cum_o = nullptr;
+  rc = [&]() -> int {
+    WDY_HIP(hipMalloc(&acc, n * 8));
+    WDY_HIP(hipMalloc(&acc_o, n * 8));
+    WDY_HIP(hipMalloc(&cum_o, n * 8));
+    // small mode writes the last acceleration per sheet, large mode per slot
+    int r = advance_impl(s, dt, nleap, s->small ? acc_o : acc, cum_o);
+    if (r) return r;
+    r = launch_desort(s, s->out_x, s->out_v, s->small ? nullptr : acc, acc_o,
+                      nullptr);
+    if (r) return r;
+    const int c = s->small ? 0 : s->cur;
+    std::vector<u64> keys(n);
+    WDY_HIP(hipMemcpyAsync(keys.data(), s->key[c], n * 8,
+                           hipMemcpyDeviceToHost, s->stream));
+    WDY_HIP(hipMemcpyAsync(ids.data(), s->idx[c], n * 4,
+                           hipMemcpyDeviceToHost, s->stream));
+    WDY_HIP(hipMemcpyAsync(x, s->out_x, n * 8, hipMemcpyDeviceToHost, s->stream));
+    WDY_HIP(hipMemcpyAsync(v, s->small ? s->vel[0] : s->out_v, n * 8,
+                           hipMemcpyDeviceToHost, s->stream));
+    if (a)
+      WDY_HIP(hipMemcpyAsync(a, acc_o, n * 8, hipMemcpyDeviceToHost, s->stream));
+    if (cumulmass)
+      WDY_HIP(hipMemcpyAsync(cumulmass, cum_o, n * 8, hipMemcpyDeviceToHost,
+                             s->stream));
+    WDY_HIP(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < n; ++i) {
+      xi[i].idx = ids[i];
+      xi[i].val = decode_pos(keys[i]);
+    }
+    float ms_ = 0.f;
+    if (hipEventElapsedTime(&ms_, s->ev0, s->ev1) != hipSuccess) ms_ = 0.f;
+    if (time_elapsed) *time_elapsed = ms_ * 1e-3;
+    return WDY_OK;
+  }();
+  if (acc) (void)hipFree(acc);
+  if (acc_o) (void)hipFree(acc_o);
+  if (cum_o) (void)hipFree(cum_o);
+  wdy_destroy(s);
+  set_err(rc);
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+// Internal declarations shared by the translation units of libwendy_amd.so.
+//
+// Device state of one simulation ("sim") is structure-of-arrays, kept in
+// sorted position order between leapfrog steps:
+//
+//   key[s][i]   uint64  order-preserving encoding of the position of the
+//                       sheet at sorted slot i (the encoding is a bijection on
+//                       non-NaN doubles with -0.0 folded into +0.0, so the key
+//                       IS the position: no separate x array exists)
+//   vel[s][i]   double  velocity of that sheet
+//   idx[s][i]   int32   original index of that sheet (only read at output)
+//   mass[s][i]  double  mass * twopiG (unequal-mass runs only)
+//
+// s in {0,1} selects one of two buffer sets; a step reads one set and writes
+// the other, so no kernel updates an array another thread of it still reads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/wendy_amd.h"
+
+namespace wdy {
+
+using u64 = uint64_t;
+using u32 = uint32_t;
+using i32 = int32_t;
+
+constexpr u64 SIGN = 0x8000000000000000ull;
+
+// Order-preserving map double -> uint64: non-negative values get the sign bit
+// set, negative values are bit-inverted.  -0.0 is folded into +0.0 first so
+// that the two zeros compare equal, as the reference comparator
+// (wendy/wendy.c:19-31) treats them.
+__host__ __device__ inline u64 encode_pos(double x) {
+  u64 b;
+  __builtin_memcpy(&b, &x, 8);
+  if (b == SIGN) b = 0;
+  return (b & SIGN) ? ~b : (b | SIGN);
+}
+
+__host__ __device__ inline double decode_pos(u64 k) {
+  u64 b = (k & SIGN) ? (k & ~SIGN) : ~k;
+  double x;
+  __builtin_memcpy(&x, &b, 8);
+  return x;
+}
+
+// Error reporting: every C-ABI entry point returns one of the WDY_E* codes and
+// leaves a message for wdy_last_error() (thread-local).
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define WDY_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return ::wdy::fail(e_ == hipErrorOutOfMemory ? WDY_ENOMEM : WDY_ERUNTIME, \
+                         "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                \
+  } while (0)
+
+#define WDY_CHECK_LAUNCH()                                                   \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess)                                                    \
+      return ::wdy::fail(WDY_ERUNTIME, "kernel launch failed: %s (%s:%d)",  \
+                         hipGetErrorString(e_), __FILE__, __LINE__);         \
+  } while (0)
+
+// --------------------------------------------------------------------------
+// Kernel timing: when enabled on a sim, every launch of a timed kernel class
+// is bracketed by two HIP events on the sim's stream.
+// --------------------------------------------------------------------------
+enum KernelClass {
+  KC_SORT_HIST = 0,   // radix: all-digit histogram
+  KC_SORT_PASS = 1,   // radix: one onesweep scatter pass
+  KC_LEAP = 2,        // fused force + kick + drift
+  KC_SCAN = 3,        // mass prefix
+  KC_OTHER = 4,       // drift-half, de-sort, tie/verify helpers
+  KC_WARM = 5,        // warm-start bucket sort + fused leap
+  KC_COUNT = 6
+};
+
+struct Timer;
+Timer *timer_create();
+void timer_destroy(Timer *t);
+void timer_begin(Timer *t, hipStream_t s, int kc);
+void timer_end(Timer *t, hipStream_t s, int kc);
+int timer_collect(Timer *t, int kc, double *ms_total, int64_t *count);
+
+// --------------------------------------------------------------------------
+// Radix sort of (uint64 key, uint32 payload) pairs: LSD, 8 passes of 8 bits,
+// one all-digit histogram pass plus one onesweep scatter pass per digit.
+// --------------------------------------------------------------------------
+struct RadixWork {
+  u32 n = 0;
+  u32 ntiles = 0;
+  u32 *hist = nullptr;       // [8][256] digit counts, then exclusive offsets
+  u32 *counters = nullptr;   // [8] tile tickets, one per pass
+  u64 *status = nullptr;     // [ntiles][256] lookback words
+  u32 epoch = 0;             // sequence number of the next pass
+};
+
+int radix_alloc(RadixWork *w, u32 n);
+void radix_free(RadixWork *w);
+// Sorts keys[0..n) ascending, stably; keys ping-pong between k0 and k1 and
+// end in k0 (8 passes).  The payload ends in p0: if identity_payload, the
+// first pass generates p = source position and p0's prior contents are
+// ignored; p1 is scratch.
+int radix_sort_pairs(RadixWork *w, hipStream_t s, Timer *tm, u64 *k0, u64 *k1,
+                     u32 *p0, u32 *p1, bool identity_payload);
+
+// --------------------------------------------------------------------------
+// Simulation state
+// --------------------------------------------------------------------------
+}  // namespace wdy
+
+struct wdy_sim {
+  int64_t n = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool equal_mass = false;
+  double m_equal = 0.0;      // mass * twopiG of every sheet (equal-mass runs)
+  double totmass = 0.0;      // numpy.sum of mass * twopiG, supplied by the host
+  double omega2 = -1.0;      // < 0: no harmonic term
+  int cur = 0;               // buffer set holding the current state
+  bool sorted_now = false;   // state is sorted (no drift since the last sort)
+  // small mode: one workgroup advances a whole output interval in LDS;
+  // key[0]/idx[0] sorted pairs, vel[0]/mass[0] in original sheet order.
+  // large mode: every array of set `cur` in sorted order, multi-kernel step.
+  bool small = false;
+
+  wdy::u64 *key[2] = {nullptr, nullptr};
+  double *vel[2] = {nullptr, nullptr};
+  wdy::i32 *idx[2] = {nullptr, nullptr};
+  double *mass[2] = {nullptr, nullptr};
+  double *cum = nullptr;         // exclusive mass prefix in sorted order
+  wdy::u32 *perm[2] = {nullptr, nullptr};
+  wdy::u64 *skey = nullptr;      // sort ping-pong partner of the key array
+  double *tile_sums = nullptr;   // mass-prefix tile partials
+  double *red = nullptr;         // reduction partials (energy, momentum)
+  double *out_x = nullptr;       // de-sorted output staging
+  double *out_v = nullptr;
+  wdy::RadixWork radix;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.0f;
+  wdy::Timer *timer = nullptr;
+  bool timing = false;
+};
+
+namespace wdy {
+// Step kernels (wdy_step.hip)
+int launch_drift(wdy_sim *s, double hd);
+int launch_sort_state(wdy_sim *s);   // sort state -> perm/keys, no payload move
+int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
+                double *cum_out);
+// de-sort the current state: x_out[j], v_out[j] (v_out may be null, and is
+// ignored in small mode, whose velocities are already in sheet order),
+// acc_out[j] = acc[slot of j] when acc is given, m_out[j] (large unequal-mass
+// mode only) when m_out is given
+int launch_desort(wdy_sim *s, double *x_out, double *v_out, const double *acc,
+                  double *acc_out, double *m_out);
+// cold sort at creation: builds the mode's layout from staged arrays
+int launch_build_state(wdy_sim *s, const double *x_slot, const i32 *id_slot,
+                       const double *v_id, const double *m_id);
+// total energy of sheets given by device arrays in any order (m_id null:
+// every mass is mq); masses are mscale * m
+int energy_core(hipStream_t st, u32 n, const double *x, const double *v,
+                const double *m, double mq, double mscale, double twopiG,
+                double omega2, double *e_out);
+// small mode (wdy_small.hip)
+u32 small_max_n(int device);
+int launch_small_advance(wdy_sim *s, double h, int nleap, double *acc_out,
+                         double *cum_out);
+int launch_energy(wdy_sim *s, double twopiG, double omega2, double *e_out);
+int launch_energy_arrays(int64_t n, const double *x, const double *v,
+                         const double *m, double twopiG, double omega2,
+                         double *e_out);
+int launch_momentum_arrays(int64_t n, const double *v, const double *m,
+                           double *p_out);
+int launch_potential_arrays(int64_t ny, const double *y, int64_t n,
+                            const double *x, const double *m, double twopiG,
+                            double omega2, double *out);
+int launch_energy_individual_arrays(int64_t n, const double *x, const double *v,
+                                    const double *m, double twopiG,
+                                    double omega2, double *out);
+}  // namespace wdy

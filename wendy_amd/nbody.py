@@ -1,0 +1,210 @@
+"""The ``nbody`` generator and the diagnostics of wendy, on the GPU.
+
+Drop-in for the reference's Python API (wendy/wendy.py:112-157,336-517):
+same names, arguments, defaults, lazy errors and yield tuples, for the
+approximate (``approx=True``) integration path.  The simulation state is
+uploaded once and stays on the GPU between yields; each ``next()`` runs one
+output interval of ``nleap`` leapfrog steps there and copies the de-sorted
+positions and velocities back into the generator's own ``x``, ``v`` arrays.
+"""
+import copy
+import ctypes
+
+import numpy
+
+from . import _lib
+
+# wendy/wendy.py:102 -- every name maps to the same device sort
+_sort_type_dict = {'quick': 0, 'merge': 1, 'tim': 2, 'qsort': 3, 'parallel': 4}
+
+
+class Sim:
+    """One device-resident approx simulation (a ``wdy_sim`` handle).
+
+    ``m`` is already multiplied by twopiG and ``totmass`` is its
+    ``numpy.sum``, as wendy/wendy.py:371,383 prepare them.
+    """
+
+    def __init__(self, x, v, m, totmass, omega2, equal_mass=None, path=None):
+        """``path``: None (by size), 'small' (one persistent workgroup per
+        output interval) or 'large' (multi-kernel step over the GPU)."""
+        self._h = None
+        lib = _lib.lib()
+        x = _lib.f64(x)
+        v = _lib.f64(v)
+        m = _lib.f64(m)
+        if not (len(x) == len(v) == len(m)):
+            raise ValueError('x, v and m must have the same length')
+        if equal_mass is None:
+            equal_mass = bool(len(m) > 0 and numpy.all(m == m[0]))
+        flags = _lib.WDY_EQUAL_MASS if equal_mass else 0
+        if path == 'large':
+            flags |= _lib.WDY_FORCE_LARGE
+        elif path == 'small':
+            flags |= _lib.WDY_FORCE_SMALL
+        elif path is not None:
+            raise ValueError("path must be None, 'small' or 'large'")
+        h = ctypes.c_void_p()
+        _lib.check(lib.wdy_create(len(x), x, v, m, float(totmass),
+                                  float(omega2), flags, ctypes.byref(h)))
+        self._h = h
+        self.n = len(x)
+        self.equal_mass = equal_mass
+
+    def advance(self, h, nleap):
+        _lib.check(_lib.lib().wdy_advance(self._h, float(h), int(nleap)))
+
+    def read(self, x_out=None, v_out=None):
+        """De-sort into ``x_out``/``v_out`` (float64, length n, C-contiguous);
+        returns the device seconds of the last ``advance``."""
+        sec = ctypes.c_double(0.)
+        for a in (x_out, v_out):
+            if a is not None and (a.dtype != numpy.float64 or a.shape != (self.n,)
+                                  or not a.flags['C_CONTIGUOUS']
+                                  or not a.flags['WRITEABLE']):
+                raise ValueError('output arrays must be writeable float64[n]')
+        xp = None if x_out is None else x_out.ctypes.data_as(ctypes.c_void_p)
+        vp = None if v_out is None else v_out.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(_lib.lib().wdy_read(self._h, xp, vp, ctypes.byref(sec)))
+        return sec.value
+
+    def energy(self, twopiG=1., omega=None):
+        """Total energy of the live device state (masses = m / twopiG)."""
+        e = ctypes.c_double(0.)
+        omega2 = -1. if omega is None else omega ** 2.
+        _lib.check(_lib.lib().wdy_energy(self._h, float(twopiG), omega2,
+                                         ctypes.byref(e)))
+        return e.value
+
+    def set_timing(self, on=True):
+        _lib.check(_lib.lib().wdy_set_timing(self._h, int(bool(on))))
+
+    def kernel_time(self, kernel_class):
+        """(summed milliseconds, launch count) of one kernel class."""
+        ms = ctypes.c_double(0.)
+        cnt = ctypes.c_int64(0)
+        _lib.check(_lib.lib().wdy_kernel_time(self._h, int(kernel_class),
+                                              ctypes.byref(ms),
+                                              ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _lib.lib().wdy_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
+          approx=False, nleap=None, sort='quick',
+          maxcoll=100000, warn_maxcoll=False,
+          full_output=False):
+    """Run a 1-D N-body simulation; generator of (x, v) at intervals dt.
+
+    Signature and defaults of wendy.nbody (wendy/wendy.py:112-115).  With
+    ``approx=True`` each output is ``nleap`` drift-kick-drift leapfrog steps
+    of size dt/nleap with exact forces, run on the GPU.  ``sort`` accepts the
+    reference's names ('quick', 'merge', 'tim', 'qsort', 'parallel'); all of
+    them use the device sort, which orders exactly tied positions by original
+    index.  Yields ``(x, v)`` or, with ``full_output``, ``(x, v, seconds)``
+    where seconds is the device time of the interval.  The yielded arrays are
+    the generator's own and are overwritten by the next ``next()``, as in the
+    reference; unlike the reference, editing them does not feed back into the
+    simulation (its state lives on the GPU).
+
+    ``approx=False`` (the exact event-driven solver) and ``ext_force`` are
+    not provided by this library: the first ``next()`` raises
+    NotImplementedError.  ``t0``, ``maxcoll`` and ``warn_maxcoll`` only matter
+    to those and are accepted for signature compatibility.
+    """
+    if approx:
+        if nleap is None:
+            raise ValueError('When approx is True, the number of leapfrog steps '
+                             'nleap= per output time step needs to be set')
+        for item in _nbody_approx(x, v, m, dt, nleap, t0=t0, sort=sort,
+                                  omega=omega, ext_force=ext_force,
+                                  twopiG=twopiG, full_output=full_output):
+            yield item
+    raise NotImplementedError(
+        'wendy_amd implements the approximate integration path only '
+        '(approx=True); the exact event-driven solver is not provided')
+
+
+def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
+                  sort='quick', twopiG=1., full_output=False):
+    """Approx path (wendy/wendy.py:336-437) with device-resident state."""
+    if ext_force is not None:
+        raise NotImplementedError('ext_force is not supported on the GPU path')
+    _sort_type_dict[sort]  # unknown names raise KeyError, as the reference
+    if omega is None:
+        omega2 = -1.
+    else:
+        omega2 = omega ** 2.
+    x = copy.copy(x)
+    v = copy.copy(v)
+    m = twopiG * copy.copy(m)
+    x = numpy.require(x, dtype=numpy.float64, requirements=['C', 'W'])
+    v = numpy.require(v, dtype=numpy.float64, requirements=['C', 'W'])
+    m = numpy.require(m, dtype=numpy.float64, requirements=['C', 'W'])
+    totmass = numpy.sum(m)
+    sim = Sim(x, v, m, totmass, omega2)
+    dt_leap = dt / nleap
+    try:
+        while True:
+            sim.advance(dt_leap, nleap)
+            seconds = sim.read(x, v)
+            if full_output:
+                yield (x, v, seconds)
+            else:
+                yield (x, v)
+    finally:
+        sim.close()
+
+
+def energy(x, v, m, twopiG=1., individual=False, omega=None):
+    """wendy.energy (wendy/wendy.py:439-475), computed on the GPU.
+
+    The total is ``sum m v^2/2 + twopiG sum_{i<j} m_i m_j |x_i - x_j|``
+    (+ ``sum m omega^2 x^2/2``), evaluated from one device sort of x and the
+    exclusive mass prefix as ``twopiG sum_i m_i x_i (2 C_i + m_i - M)``.
+    """
+    lib = _lib.lib()
+    x = _lib.f64(x)
+    v = _lib.f64(v)
+    m = _lib.f64(m)
+    omega2 = -1. if omega is None else omega ** 2.
+    if individual:
+        out = numpy.empty(len(x))
+        _lib.check(lib.wdy_energy_individual_arrays(len(x), x, v, m,
+                                                    float(twopiG), omega2, out))
+        return out
+    e = ctypes.c_double(0.)
+    _lib.check(lib.wdy_energy_arrays(len(x), x, v, m, float(twopiG), omega2,
+                                     ctypes.byref(e)))
+    return e.value
+
+
+def momentum(v, m):
+    """wendy.momentum (wendy/wendy.py:477-491): sum m v, on the GPU."""
+    v = _lib.f64(v)
+    m = _lib.f64(m)
+    p = ctypes.c_double(0.)
+    _lib.check(_lib.lib().wdy_momentum_arrays(len(v), v, m, ctypes.byref(p)))
+    return p.value
+
+
+def potential(y, x, v, m, twopiG=1., omega=None):
+    """wendy.potential (wendy/wendy.py:493-517) at points y, on the GPU."""
+    y = _lib.f64(numpy.atleast_1d(y))
+    x = _lib.f64(x)
+    m = _lib.f64(m)
+    out = numpy.empty(len(y))
+    omega2 = -1. if omega is None else omega ** 2.
+    _lib.check(_lib.lib().wdy_potential_arrays(len(y), y, len(x), x, m,
+                                               float(twopiG), omega2, out))
+    return out
