@@ -45,6 +45,11 @@ extern "C" {
  * large path's code at small n. */
 #define WDY_FORCE_LARGE 2
 #define WDY_FORCE_SMALL 4
+/* The large path re-sorts each leapfrog step with one warm-start kernel
+ * (value buckets seeded by the previous steps' order) and falls back to the
+ * full radix sort when a bucket outgrows its limits.  This flag keeps the
+ * large path on the radix sort alone (for comparison and testing). */
+#define WDY_NO_WARM 8
 
 typedef struct wdy_sim wdy_sim;
 
@@ -91,6 +96,23 @@ int wdy_energy(wdy_sim *sim, double twopiG, double omega2, double *e_out);
 
 /* Free the simulation's device buffers and stream. */
 int wdy_destroy(wdy_sim *sim);
+
+/* Warm-start tuning of a large-path sim: the target sheets per value bucket
+ * (WDY_BUCKET_MIN .. 6144 for equal masses, .. 4608 otherwise) and the
+ * window of neighbouring buckets a sheet may move into per step
+ * (0 .. WDY_WINDOW_MAX; farther movers travel through an overflow list).
+ * Takes effect from the next output interval, which runs on the radix path
+ * to rebuild the bucket layout.  No reference counterpart (the reference
+ * re-sorts from scratch each step). */
+#define WDY_BUCKET_MIN 32
+#define WDY_WINDOW_MAX 64
+int wdy_tune(wdy_sim *sim, int bucket_target, int window);
+
+/* Leapfrog steps taken so far by the warm-start kernel and by the radix
+ * path, and the number of output intervals redone on the radix path after a
+ * warm-start failure.  No reference counterpart. */
+int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
+                   int64_t *redos);
 
 /* Per-kernel-class device time, for benchmarks.  wdy_set_timing(sim, 1)
  * brackets every subsequent launch with HIP events on the sim's stream;

@@ -64,7 +64,7 @@ def test_create_rejects_bad_arguments_before_touching_device():
     assert b'non-finite' in lib.wdy_last_error()
     y = numpy.array([0., 1., 2.])
     assert lib.wdy_create(0, y, y, y, 1., -1., 0, ctypes.byref(h)) == _lib.WDY_EINVAL
-    assert lib.wdy_create(3, y, y, y, 1., -1., 8, ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert lib.wdy_create(3, y, y, y, 1., -1., 64, ctypes.byref(h)) == _lib.WDY_EINVAL
     # WDY_EQUAL_MASS with unequal masses is refused
     assert lib.wdy_create(3, y, y, y, 1., -1., 1, ctypes.byref(h)) == _lib.WDY_EINVAL
     with pytest.raises(ValueError):

@@ -1,0 +1,157 @@
+"""The large path's warm-start step (one kernel per leapfrog step) against the
+oracle, including its corner paths, reached at small N by shrinking its
+limits (as the reference's CI shrinks the parallel-sort thresholds,
+.github/workflows/build.yml:30): tiny buckets, a zero window (every sheet that
+changes bucket travels through the overflow list), and forced failures that
+make the library redo an interval on the radix path.
+
+Bars as in test_gpu_parity: bitwise for power-of-two equal masses, else
+||.||_inf relative 1e-10."""
+import numpy
+import pytest
+
+import wendy_amd
+from conftest import bits_equal, rel_inf
+from oracle import nbody_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def make(n, seed, mass='equal', vshift=None):
+    return O.sech2_disk(n, seed=seed, mass='equal' if mass == 'equal'
+                        else ('lognormal', 0.5), vshift=vshift)
+
+
+def exact_for(m):
+    return bool(numpy.all(m == m[0]) and numpy.frexp(m[0])[0] == 0.5)
+
+
+def check(got, want, exact, what):
+    if exact:
+        assert bits_equal(got[0], want[0]), (what, rel_inf(got[0], want[0]))
+        assert bits_equal(got[1], want[1]), (what, rel_inf(got[1], want[1]))
+    else:
+        assert rel_inf(got[0], want[0]) <= TOL, (what, rel_inf(got[0], want[0]))
+        assert rel_inf(got[1], want[1]) <= TOL, (what, rel_inf(got[1], want[1]))
+
+
+def run_both(x, v, m, dt, nleap, nout, omega=None, tune=None, warm=True,
+             hs=None):
+    """Run the large path and the oracle side by side; returns the sim's
+    path statistics.  ``hs`` optionally gives each output's leapfrog h."""
+    omega2 = -1. if omega is None else omega ** 2.
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), omega2, path='large', warm=warm)
+    if tune:
+        sim.tune(*tune)
+    st = O.ApproxState(x, v, m, dt, nleap, omega=omega)
+    xo = numpy.empty(len(x))
+    vo = numpy.empty(len(x))
+    exact = exact_for(m)
+    try:
+        for k in range(nout):
+            h = dt / nleap if hs is None else hs[k]
+            st.h = h
+            sim.advance(h, nleap)
+            sim.read(xo, vo)
+            check((xo, vo), st.onestep(), exact, 'output %d' % (k + 1))
+        return sim.path_stats()
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize('n,mass,omega', [
+    (32768, 'equal', 0.1),
+    (20000, 'equal', None),
+    (30000, 'lognormal', None),
+    (25000, 'lognormal', 0.3),
+])
+def test_warm_matches_oracle(gpu_lib, n, mass, omega):
+    print('warm', n, mass, omega, flush=True)
+    x, v, m = make(n, n, mass)
+    warm, cold, redo = run_both(x, v, m, 0.01, 5, 4, omega=omega)
+    assert warm == 15 and cold == 5 and redo == 0
+
+
+def test_warm_bulk_flow(gpu_lib):
+    """The large unequal-mass workload's shape: +0.3 bulk velocity on x > 0."""
+    x, v, m = make(40000, 7, 'lognormal', vshift=0.3)
+    warm, cold, redo = run_both(x, v, m, 0.005, 10, 3)
+    assert warm == 20 and redo == 0
+
+
+@pytest.mark.parametrize('tune', [(64, 2), (32, 1), (100, 5)])
+def test_warm_many_small_buckets(gpu_lib, tune):
+    x, v, m = make(6000, 1, 'lognormal')
+    warm, _, _ = run_both(x, v, m, 0.02, 4, 4, tune=tune)
+    assert warm > 0
+
+
+@pytest.mark.parametrize('mass', ['equal', 'lognormal'])
+def test_warm_zero_window_uses_overflow_list(gpu_lib, mass):
+    """Window 0: every sheet that changes bucket is a far mover."""
+    x, v, m = make(8192, 2, mass)
+    warm, _, redo = run_both(x, v, m, 0.02, 4, 4, tune=(64, 0))
+    assert warm > 0
+
+
+def test_warm_redo_after_overflow(gpu_lib):
+    """Far movers beyond the overflow list's capacity make the warm interval
+    fail on the device; the library rewinds, redoes it on the radix path,
+    widens the window and goes on -- with unchanged results."""
+    n = 1 << 18
+    x, v, m = make(n, 3)
+    warm, cold, redo = run_both(x, v, m, 0.02, 5, 5, tune=(32, 0))
+    assert redo >= 1
+
+
+def test_warm_capacity_limit_bucket(gpu_lib):
+    """Buckets targeted at the LDS capacity: any that outgrows it fails the
+    interval, which is redone; results are unchanged either way."""
+    x, v, m = make(1 << 17, 4)
+    warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, tune=(6144, 8))
+    assert warm + cold == 20
+
+
+def test_warm_h_change_between_outputs(gpu_lib):
+    """A different h invalidates the anticipated half drift: that interval
+    runs on the radix path, then the warm path resumes."""
+    x, v, m = make(20000, 5)
+    hs = [0.002, 0.002, 0.001, 0.001, 0.002]
+    warm, cold, redo = run_both(x, v, m, 0.01, 5, 5, hs=hs)
+    assert cold == 15 and warm == 10
+
+
+def test_warm_equals_radix_path(gpu_lib):
+    """Equal masses: the warm kernel and the radix path compute the same
+    ranks, so their trajectories agree bit for bit even for masses that are
+    not powers of two."""
+    x, v, m = make(50000, 6)
+    outs = []
+    for warm in (True, False):
+        sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path='large', warm=warm)
+        xo = numpy.empty(len(x))
+        vo = numpy.empty(len(x))
+        for _ in range(3):
+            sim.advance(0.001, 6)
+        sim.read(xo, vo)
+        outs.append((xo, vo))
+        sim.close()
+    assert bits_equal(outs[0][0], outs[1][0]) and bits_equal(outs[0][1], outs[1][1])
+
+
+def test_warm_unequal_deterministic(gpu_lib):
+    x, v, m = make(60000, 8, 'lognormal')
+    res = []
+    for _ in range(2):
+        sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large')
+        xo = numpy.empty(len(x))
+        vo = numpy.empty(len(x))
+        for _ in range(3):
+            sim.advance(0.002, 5)
+        sim.read(xo, vo)
+        res.append((xo, vo))
+        assert sim.path_stats()[0] == 10
+        sim.close()
+    assert bits_equal(res[0][0], res[1][0]) and bits_equal(res[0][1], res[1][1])

@@ -26,11 +26,13 @@ WDY_EUNSUPPORTED = -6
 WDY_EQUAL_MASS = 1
 WDY_FORCE_LARGE = 2
 WDY_FORCE_SMALL = 4
+WDY_NO_WARM = 8
 
 # every C symbol include/wendy_amd.h declares
 EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_create', 'wdy_advance', 'wdy_read', 'wdy_energy',
-            'wdy_destroy', 'wdy_set_timing', 'wdy_kernel_time',
+            'wdy_destroy', 'wdy_tune', 'wdy_path_stats',
+            'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
             '_wendy_nbody_approx_onestep')
@@ -71,6 +73,10 @@ def _declare(lib):
     lib.wdy_energy.argtypes = [S, ctypes.c_double, ctypes.c_double, _c_double_p]
     lib.wdy_destroy.restype = ctypes.c_int
     lib.wdy_destroy.argtypes = [S]
+    lib.wdy_tune.restype = ctypes.c_int
+    lib.wdy_tune.argtypes = [S, ctypes.c_int, ctypes.c_int]
+    lib.wdy_path_stats.restype = ctypes.c_int
+    lib.wdy_path_stats.argtypes = [S, _c_int64_p, _c_int64_p, _c_int64_p]
     lib.wdy_set_timing.restype = ctypes.c_int
     lib.wdy_set_timing.argtypes = [S, ctypes.c_int]
     lib.wdy_kernel_time.restype = ctypes.c_int

@@ -114,13 +114,15 @@ __global__ void k_iota(u32 n, i32 *__restrict__ idx) {
 
 static void free_sim(wdy_sim *s) {
   if (!s) return;
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < 3; ++b) {
     if (s->key[b]) (void)hipFree(s->key[b]);
     if (s->vel[b]) (void)hipFree(s->vel[b]);
     if (s->idx[b]) (void)hipFree(s->idx[b]);
     if (s->mass[b]) (void)hipFree(s->mass[b]);
-    if (s->perm[b]) (void)hipFree(s->perm[b]);
   }
+  for (int b = 0; b < 2; ++b)
+    if (s->perm[b]) (void)hipFree(s->perm[b]);
+  warm_free(s);
   if (s->skey) (void)hipFree(s->skey);
   if (s->cum) (void)hipFree(s->cum);
   if (s->tile_sums) (void)hipFree(s->tile_sums);
@@ -147,7 +149,8 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
   if (n < 1 || n >= (int64_t(1) << 30))
     return fail(WDY_EINVAL, "wdy_create: n=%lld outside [1, 2^30)", (long long)n);
   if (!x_slot || !v_id || !m_id) return fail(WDY_EINVAL, "wdy_create: NULL array");
-  if (flags & ~(WDY_EQUAL_MASS | WDY_FORCE_LARGE | WDY_FORCE_SMALL))
+  if (flags & ~(WDY_EQUAL_MASS | WDY_FORCE_LARGE | WDY_FORCE_SMALL |
+                WDY_NO_WARM))
     return fail(WDY_EINVAL, "wdy_create: unknown flags 0x%x", flags);
   if ((flags & WDY_FORCE_LARGE) && (flags & WDY_FORCE_SMALL))
     return fail(WDY_EINVAL, "wdy_create: WDY_FORCE_LARGE and WDY_FORCE_SMALL");
@@ -190,15 +193,17 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     s->totmass = totmass;
     s->omega2 = omega2 < 0.0 ? -1.0 : omega2;
     const size_t N = (size_t)n;
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < 3; ++b) {
+      if (s->small && b == 2) break;
       WDY_HIP(hipMalloc(&s->key[b], N * sizeof(u64)));
       WDY_HIP(hipMalloc(&s->idx[b], N * sizeof(i32)));
-      WDY_HIP(hipMalloc(&s->perm[b], N * sizeof(u32)));
       if (b == 0 || !s->small) {
         WDY_HIP(hipMalloc(&s->vel[b], N * sizeof(double)));
         if (!eq) WDY_HIP(hipMalloc(&s->mass[b], N * sizeof(double)));
       }
     }
+    for (int b = 0; b < 2; ++b)
+      WDY_HIP(hipMalloc(&s->perm[b], N * sizeof(u32)));
     WDY_HIP(hipMalloc(&s->skey, N * sizeof(u64)));
     WDY_HIP(hipMalloc(&s->cum, N * sizeof(double)));
     WDY_HIP(hipMalloc(&s->tile_sums, (N / 4096 + 2) * sizeof(double)));
@@ -225,6 +230,14 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     }
     r = launch_build_state(s, s->out_x, s->idx[1], s->out_v, s->cum);
     if (r) return r;
+    // warm-start step for the large path, unless disabled; it needs
+    // non-negative masses (fixed-point prefix)
+    bool warm = !s->small && !(flags & WDY_NO_WARM);
+    for (int64_t i = 0; warm && i < n; ++i) warm = m_id[i] >= 0.0;
+    if (warm) {
+      r = warm_configure(s, eq ? 4096u : 3072u, 8u);
+      if (r) return r;
+    }
     WDY_HIP(hipStreamSynchronize(s->stream));
     return WDY_OK;
   }();
@@ -244,14 +257,35 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
     int rc = launch_small_advance(s, h, nleap, acc_last, cum_last);
     if (rc) return rc;
   } else {
+    // warm interval when the state is partitioned for this h and nothing
+    // asks for the last force; on a device-side failure it rewinds and the
+    // interval is redone on the radix path below
+    if (!acc_last && !cum_last && warm_ready_for(s, h)) {
+      bool failed = false;
+      int rc = warm_interval(s, h, nleap, &failed);
+      if (rc) return rc;
+      if (!failed) {
+        s->warm_steps += nleap;
+        WDY_HIP(hipEventRecord(s->ev1, s->stream));
+        return WDY_OK;
+      }
+      s->redos += 1;
+    }
     int rc = launch_drift(s, h / 2.);
     if (rc) return rc;
-    for (int ii = 0; ii < nleap - 1; ++ii) {
-      rc = launch_leap(s, h, h, nullptr, nullptr);
+    u32 T = 0, nb = 0;
+    u64 *spl = s->warm ? warm_cold_splitters(s, &T, &nb) : nullptr;
+    for (int ii = 0; ii < nleap; ++ii) {
+      const bool last = ii == nleap - 1;
+      rc = launch_leap(s, h, last ? h / 2. : h, last ? acc_last : nullptr,
+                       last ? cum_last : nullptr, last ? spl : nullptr, T, nb);
       if (rc) return rc;
     }
-    rc = launch_leap(s, h, h / 2., acc_last, cum_last);
-    if (rc) return rc;
+    s->cold_steps += nleap;
+    if (s->warm) {
+      rc = warm_partition_after_cold(s, h / 2.);
+      if (rc) return rc;
+    }
   }
   WDY_HIP(hipEventRecord(s->ev1, s->stream));
   return WDY_OK;
@@ -332,6 +366,29 @@ int wdy_destroy(wdy_sim *sim) {
   (void)hipSetDevice(sim->device);
   (void)hipStreamSynchronize(sim->stream);
   free_sim(sim);
+  return WDY_OK;
+}
+
+int wdy_tune(wdy_sim *sim, int bucket_target, int window) {
+  if (int rc = check_sim(sim, "wdy_tune")) return rc;
+  if (sim->small || !sim->warm)
+    return fail(WDY_EUNSUPPORTED, "wdy_tune: the sim has no warm-start path");
+  if (bucket_target < (int)WDY_BUCKET_MIN ||
+      bucket_target > (int)warm_capacity(sim->equal_mass) || window < 0 ||
+      window > WDY_WINDOW_MAX)
+    return fail(WDY_EINVAL, "wdy_tune: bucket_target %d / window %d out of range",
+                bucket_target, window);
+  WDY_HIP(hipSetDevice(sim->device));
+  WDY_HIP(hipStreamSynchronize(sim->stream));
+  return warm_configure(sim, (u32)bucket_target, (u32)window);
+}
+
+int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
+                   int64_t *redos) {
+  if (int rc = check_sim(sim, "wdy_path_stats")) return rc;
+  if (warm_steps) *warm_steps = sim->warm_steps;
+  if (cold_steps) *cold_steps = sim->cold_steps;
+  if (redos) *redos = sim->redos;
   return WDY_OK;
 }
 
@@ -447,8 +504,9 @@ void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
     eq = eq && (m[i] == m[0]);
   }
   wdy_sim *s = nullptr;
+  // the reference hands xi back in sorted order, which the radix path keeps
   int rc = create_impl(N, xs.data(), ids.data(), v, m, totmass, omega2,
-                       eq ? WDY_EQUAL_MASS : 0, &s);
+                       (eq ? WDY_EQUAL_MASS : 0) | WDY_NO_WARM, &s);
   if (rc) return set_err(rc);
   double *acc = nullptr, *acc_o = nullptr, *cum_o = nullptr;
   rc = [&]() -> int {

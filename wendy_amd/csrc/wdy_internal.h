@@ -117,6 +117,7 @@ int radix_sort_pairs(RadixWork *w, hipStream_t s, Timer *tm, u64 *k0, u64 *k1,
 // --------------------------------------------------------------------------
 // Simulation state
 // --------------------------------------------------------------------------
+struct WarmCtx;  // warm-start step metadata (wdy_warm.hip)
 }  // namespace wdy
 
 struct wdy_sim {
@@ -131,13 +132,14 @@ struct wdy_sim {
   bool sorted_now = false;   // state is sorted (no drift since the last sort)
   // small mode: one workgroup advances a whole output interval in LDS;
   // key[0]/idx[0] sorted pairs, vel[0]/mass[0] in original sheet order.
-  // large mode: every array of set `cur` in sorted order, multi-kernel step.
+  // large mode: three buffer sets; the current one holds every sheet once,
+  // in sorted order after a radix step or in the warm-step segment layout.
   bool small = false;
 
-  wdy::u64 *key[2] = {nullptr, nullptr};
-  double *vel[2] = {nullptr, nullptr};
-  wdy::i32 *idx[2] = {nullptr, nullptr};
-  double *mass[2] = {nullptr, nullptr};
+  wdy::u64 *key[3] = {nullptr, nullptr, nullptr};
+  double *vel[3] = {nullptr, nullptr, nullptr};
+  wdy::i32 *idx[3] = {nullptr, nullptr, nullptr};
+  double *mass[3] = {nullptr, nullptr, nullptr};
   double *cum = nullptr;         // exclusive mass prefix in sorted order
   wdy::u32 *perm[2] = {nullptr, nullptr};
   wdy::u64 *skey = nullptr;      // sort ping-pong partner of the key array
@@ -146,9 +148,10 @@ struct wdy_sim {
   double *out_x = nullptr;       // de-sorted output staging
   double *out_v = nullptr;
   wdy::RadixWork radix;
+  wdy::WarmCtx *warm = nullptr;  // null: the large path uses radix steps only
 
+  int64_t warm_steps = 0, cold_steps = 0, redos = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  float last_ms = 0.0f;
   wdy::Timer *timer = nullptr;
   bool timing = false;
 };
@@ -157,8 +160,10 @@ namespace wdy {
 // Step kernels (wdy_step.hip)
 int launch_drift(wdy_sim *s, double hd);
 int launch_sort_state(wdy_sim *s);   // sort state -> perm/keys, no payload move
+// radix-path leapfrog step: sort, force, kick, drift into the next buffer
+// set; spl_out (may be null) receives the sorted keys at ranks k*T
 int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
-                double *cum_out);
+                double *cum_out, u64 *spl_out, u32 T, u32 nb);
 // de-sort the current state: x_out[j], v_out[j] (v_out may be null, and is
 // ignored in small mode, whose velocities are already in sheet order),
 // acc_out[j] = acc[slot of j] when acc is given, m_out[j] (large unequal-mass
@@ -178,6 +183,14 @@ u32 small_max_n(int device);
 int launch_small_advance(wdy_sim *s, double h, int nleap, double *acc_out,
                          double *cum_out);
 int launch_energy(wdy_sim *s, double twopiG, double omega2, double *e_out);
+// warm-start step (wdy_warm.hip)
+int warm_configure(wdy_sim *s, u32 T, u32 W);
+void warm_free(wdy_sim *s);
+bool warm_ready_for(wdy_sim *s, double h);
+u64 *warm_cold_splitters(wdy_sim *s, u32 *T, u32 *nb);
+int warm_partition_after_cold(wdy_sim *s, double hp);
+int warm_interval(wdy_sim *s, double h, int nleap, bool *failed);
+u32 warm_capacity(bool equal_mass);
 int launch_energy_arrays(int64_t n, const double *x, const double *v,
                          const double *m, double twopiG, double omega2,
                          double *e_out);

@@ -93,8 +93,9 @@ __global__ __launch_bounds__(BLOCK) void k_leap(
     const double *__restrict__ mass_old, const double *__restrict__ cum,
     u64 *__restrict__ knew, double *__restrict__ vel_new,
     i32 *__restrict__ idx_new, double *__restrict__ mass_new,
-    double *__restrict__ acc_out, double *__restrict__ cum_out, double mq,
-    double totmass, double omega2, double hk, double hd) {
+    double *__restrict__ acc_out, double *__restrict__ cum_out,
+    u64 *__restrict__ spl_out, u32 T, u32 nb, double mq, double totmass,
+    double omega2, double hk, double hd) {
   const u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const u64 k = ks[i];
@@ -120,6 +121,10 @@ __global__ __launch_bounds__(BLOCK) void k_leap(
   if (!EQ) mass_new[i] = m;
   if (acc_out) acc_out[i] = a;
   if (cum_out) cum_out[i] = C;
+  if (spl_out) {  // sorted (pre-drift) keys at ranks k*T: warm-step splitters
+    if (i % T == 0) spl_out[i / T] = i == 0 ? 0ull : k;
+    if (i == n - 1) spl_out[nb] = ~0ull;
+  }
 }
 
 // Cold-sort layout builder.  ks[i] sorted keys, perm[i] their staged slot,
@@ -405,11 +410,11 @@ int launch_sort_state(wdy_sim *s) {
 }
 
 int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
-                double *cum_out) {
+                double *cum_out, u64 *spl_out, u32 T, u32 nb) {
   const u32 n = (u32)s->n;
   int rc = launch_sort_state(s);
   if (rc) return rc;
-  const int c = s->cur, o = 1 - s->cur;
+  const int c = s->cur, o = (s->cur + 1) % 3;
   if (!s->equal_mass) {
     rc = mass_prefix(s, s->mass[c], s->perm[0], s->cum);
     if (rc) return rc;
@@ -419,8 +424,8 @@ int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
 #define WDY_LEAP(EQ, HARM)                                                      \
   k_leap<EQ, HARM><<<grid_for(n), BLOCK, 0, s->stream>>>(                       \
       n, s->key[c], s->perm[0], s->vel[c], s->idx[c], s->mass[c], s->cum,       \
-      s->key[o], s->vel[o], s->idx[o], s->mass[o], acc_out, cum_out,            \
-      s->m_equal, s->totmass, s->omega2, hk, hd)
+      s->key[o], s->vel[o], s->idx[o], s->mass[o], acc_out, cum_out, spl_out,   \
+      T ? T : 1u, nb, s->m_equal, s->totmass, s->omega2, hk, hd)
   if (s->equal_mass) {
     if (harm) WDY_LEAP(true, true); else WDY_LEAP(true, false);
   } else {

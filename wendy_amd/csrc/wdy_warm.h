@@ -1,0 +1,61 @@
+// Shared declarations of the warm-start leapfrog step (wdy_warm.hip).
+#pragma once
+
+#include "wdy_internal.h"
+
+namespace wdy {
+
+constexpr int WARM_THREADS = 1024;     // threads per bucket workgroup
+constexpr int WARM_NQ = 4096;          // position bins of the in-LDS sort
+constexpr int WARM_WMAX = 64;          // largest window (buckets either side)
+constexpr int WARM_CAP_EQ = 6144;      // LDS bucket capacity, equal masses
+constexpr int WARM_CAP_NEQ = 4608;     // LDS bucket capacity, unequal masses
+constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry
+constexpr u32 WARM_T_MIN = 32;
+
+// Segment table and overflow list written by one step, read by the next.
+// Source bucket t occupies [boff[t], boff[t] + count) of the state arrays,
+// partitioned into S+1 contiguous segments: segment d (< S) holds the sheets
+// whose next bucket is t + d - W; segment S holds the far movers, which are
+// also copied into the overflow list with their destination bucket.
+struct WarmMeta {
+  u32 *segcnt = nullptr;  // [nb * (S + 1)]
+  u32 *segoff = nullptr;  // [nb * (S + 1)] offset of the segment in its bucket
+  u32 *boff = nullptr;    // [nb + 1] region start of each bucket
+  u64 *spl = nullptr;     // [nb + 1] splitters this step produced
+  u32 *ctr = nullptr;     // [4] ticket, done, overflow count
+  u64 *okey = nullptr;    // [WARM_OVF_CAP] overflow records
+  double *ov = nullptr;
+  i32 *oid = nullptr;
+  double *om = nullptr;
+  u32 *odst = nullptr;
+};
+
+struct WarmArgs {
+  u32 n, nb, T, W, S;
+  const u64 *kin;
+  const double *vin;
+  const i32 *iin;
+  const double *min;
+  u64 *kout;
+  double *vout;
+  i32 *iout;
+  double *mout;
+  WarmMeta rd, wr;
+  u64 *status;  // [nb * 3] look-back words
+  u32 *err;     // [2] epoch of the first failing launch, failure kinds
+  u32 epoch;
+  double mq, totmass, omega2, h, pre, hd, hp, fscale, finv;
+};
+
+// failure kinds (err[1] bits)
+constexpr u32 WARM_ERR_CAPACITY = 1;  // a bucket exceeded the LDS capacity
+constexpr u32 WARM_ERR_OVF = 2;       // the overflow list was full
+constexpr u32 WARM_ERR_LOVF = 4;      // too many far movers into one bucket
+constexpr u32 WARM_ERR_STALL = 8;     // look-back gave up (bounded wait)
+
+int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
+                     bool harm, int mode);
+int warm_reset_launch(hipStream_t st, u32 *ctr_a, u32 *ctr_b, u32 *err);
+
+}  // namespace wdy

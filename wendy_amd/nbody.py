@@ -25,9 +25,11 @@ class Sim:
     ``numpy.sum``, as wendy/wendy.py:371,383 prepare them.
     """
 
-    def __init__(self, x, v, m, totmass, omega2, equal_mass=None, path=None):
+    def __init__(self, x, v, m, totmass, omega2, equal_mass=None, path=None,
+                 warm=True):
         """``path``: None (by size), 'small' (one persistent workgroup per
-        output interval) or 'large' (multi-kernel step over the GPU)."""
+        output interval) or 'large' (whole-GPU steps).  ``warm=False`` keeps
+        the large path on the full radix sort every step."""
         self._h = None
         lib = _lib.lib()
         x = _lib.f64(x)
@@ -44,6 +46,8 @@ class Sim:
             flags |= _lib.WDY_FORCE_SMALL
         elif path is not None:
             raise ValueError("path must be None, 'small' or 'large'")
+        if not warm:
+            flags |= _lib.WDY_NO_WARM
         h = ctypes.c_void_p()
         _lib.check(lib.wdy_create(len(x), x, v, m, float(totmass),
                                   float(omega2), flags, ctypes.byref(h)))
@@ -75,6 +79,18 @@ class Sim:
         _lib.check(_lib.lib().wdy_energy(self._h, float(twopiG), omega2,
                                          ctypes.byref(e)))
         return e.value
+
+    def tune(self, bucket_target, window):
+        """Warm-start bucket target and window (large path); applies from the
+        next output interval."""
+        _lib.check(_lib.lib().wdy_tune(self._h, int(bucket_target), int(window)))
+
+    def path_stats(self):
+        """(warm-start steps, radix steps, redone intervals) so far."""
+        a, b, c = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.check(_lib.lib().wdy_path_stats(self._h, ctypes.byref(a),
+                                             ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def set_timing(self, on=True):
         _lib.check(_lib.lib().wdy_set_timing(self._h, int(bool(on))))
