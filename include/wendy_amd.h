@@ -97,16 +97,22 @@ int wdy_energy(wdy_sim *sim, double twopiG, double omega2, double *e_out);
 /* Free the simulation's device buffers and stream. */
 int wdy_destroy(wdy_sim *sim);
 
-/* Warm-start tuning of a large-path sim: the target sheets per value bucket
- * (WDY_BUCKET_MIN .. 6144 for equal masses, .. 4608 otherwise) and the
- * window of neighbouring buckets a sheet may move into per step
- * (0 .. WDY_WINDOW_MAX; farther movers travel through an overflow list).
+/* Warm-start tuning of a large-path sim.
+ *   shape          tile shape of the warm kernel (-1 keeps the current one):
+ *                  0: 1024 threads, buckets of up to 4096 sheets (default)
+ *                  1: 512 threads, up to 2048;  2: 256 threads, up to 1024;
+ *                  3: 512 threads, up to 1024
+ *   bucket_target  sheets per value bucket, WDY_BUCKET_MIN .. the shape's
+ *                  capacity; 0 picks 3/4 of the capacity
+ *   window         neighbouring buckets a sheet may move into per step
+ *                  (0 .. WDY_WINDOW_MAX, default 8); farther movers travel
+ *                  through an overflow list
  * Takes effect from the next output interval, which runs on the radix path
  * to rebuild the bucket layout.  No reference counterpart (the reference
  * re-sorts from scratch each step). */
 #define WDY_BUCKET_MIN 32
 #define WDY_WINDOW_MAX 64
-int wdy_tune(wdy_sim *sim, int bucket_target, int window);
+int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window);
 
 /* Leapfrog steps taken so far by the warm-start kernel and by the radix
  * path, and the number of output intervals redone on the radix path after a

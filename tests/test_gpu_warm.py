@@ -110,7 +110,7 @@ def test_warm_capacity_limit_bucket(gpu_lib):
     """Buckets targeted at the LDS capacity: any that outgrows it fails the
     interval, which is redone; results are unchanged either way."""
     x, v, m = make(1 << 17, 4)
-    warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, tune=(6144, 8))
+    warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, tune=(4096, 8))
     assert warm + cold == 20
 
 

@@ -74,7 +74,7 @@ def _declare(lib):
     lib.wdy_destroy.restype = ctypes.c_int
     lib.wdy_destroy.argtypes = [S]
     lib.wdy_tune.restype = ctypes.c_int
-    lib.wdy_tune.argtypes = [S, ctypes.c_int, ctypes.c_int]
+    lib.wdy_tune.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.wdy_path_stats.restype = ctypes.c_int
     lib.wdy_path_stats.argtypes = [S, _c_int64_p, _c_int64_p, _c_int64_p]
     lib.wdy_set_timing.restype = ctypes.c_int

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "wdy_internal.h"
+#include "wdy_warm.h"
 
 namespace wdy {
 
@@ -235,7 +236,7 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     bool warm = !s->small && !(flags & WDY_NO_WARM);
     for (int64_t i = 0; warm && i < n; ++i) warm = m_id[i] >= 0.0;
     if (warm) {
-      r = warm_configure(s, eq ? 4096u : 3072u, 8u);
+      r = warm_configure(s, WARM_SHAPE_DEFAULT, 0u, 8u);
       if (r) return r;
     }
     WDY_HIP(hipStreamSynchronize(s->stream));
@@ -369,18 +370,21 @@ int wdy_destroy(wdy_sim *sim) {
   return WDY_OK;
 }
 
-int wdy_tune(wdy_sim *sim, int bucket_target, int window) {
+int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {
   if (int rc = check_sim(sim, "wdy_tune")) return rc;
   if (sim->small || !sim->warm)
     return fail(WDY_EUNSUPPORTED, "wdy_tune: the sim has no warm-start path");
-  if (bucket_target < (int)WDY_BUCKET_MIN ||
-      bucket_target > (int)warm_capacity(sim->equal_mass) || window < 0 ||
-      window > WDY_WINDOW_MAX)
+  if (shape < -1 || shape >= WARM_NSHAPES)
+    return fail(WDY_EINVAL, "wdy_tune: shape %d out of range", shape);
+  const int cap = warm_shape(shape >= 0 ? shape : warm_current_shape(sim)).cap;
+  if ((bucket_target != 0 &&
+       (bucket_target < (int)WDY_BUCKET_MIN || bucket_target > cap)) ||
+      window < 0 || window > WDY_WINDOW_MAX)
     return fail(WDY_EINVAL, "wdy_tune: bucket_target %d / window %d out of range",
                 bucket_target, window);
   WDY_HIP(hipSetDevice(sim->device));
   WDY_HIP(hipStreamSynchronize(sim->stream));
-  return warm_configure(sim, (u32)bucket_target, (u32)window);
+  return warm_configure(sim, shape, (u32)bucket_target, (u32)window);
 }
 
 int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,

@@ -184,13 +184,13 @@ int launch_small_advance(wdy_sim *s, double h, int nleap, double *acc_out,
                          double *cum_out);
 int launch_energy(wdy_sim *s, double twopiG, double omega2, double *e_out);
 // warm-start step (wdy_warm.hip)
-int warm_configure(wdy_sim *s, u32 T, u32 W);
+int warm_configure(wdy_sim *s, int shape, u32 T, u32 W);
 void warm_free(wdy_sim *s);
 bool warm_ready_for(wdy_sim *s, double h);
 u64 *warm_cold_splitters(wdy_sim *s, u32 *T, u32 *nb);
 int warm_partition_after_cold(wdy_sim *s, double hp);
 int warm_interval(wdy_sim *s, double h, int nleap, bool *failed);
-u32 warm_capacity(bool equal_mass);
+int warm_current_shape(wdy_sim *s);
 int launch_energy_arrays(int64_t n, const double *x, const double *v,
                          const double *m, double twopiG, double omega2,
                          double *e_out);

@@ -5,11 +5,16 @@
 
 namespace wdy {
 
-constexpr int WARM_THREADS = 1024;     // threads per bucket workgroup
-constexpr int WARM_NQ = 4096;          // position bins of the in-LDS sort
 constexpr int WARM_WMAX = 64;          // largest window (buckets either side)
-constexpr int WARM_CAP_EQ = 6144;      // LDS bucket capacity, equal masses
-constexpr int WARM_CAP_NEQ = 4608;     // LDS bucket capacity, unequal masses
+// Tile shapes the warm kernel is compiled for: threads per workgroup and the
+// most sheets one bucket may hold (its LDS capacity).  The default bucket
+// target is 3/4 of the capacity.
+struct WarmShape {
+  int threads, cap;
+};
+constexpr int WARM_NSHAPES = 4;
+constexpr int WARM_SHAPE_DEFAULT = 0;
+const WarmShape &warm_shape(int v);
 constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry
 constexpr u32 WARM_T_MIN = 32;
 
@@ -23,7 +28,6 @@ struct WarmMeta {
   u32 *segoff = nullptr;  // [nb * (S + 1)] offset of the segment in its bucket
   u32 *boff = nullptr;    // [nb + 1] region start of each bucket
   u64 *spl = nullptr;     // [nb + 1] splitters this step produced
-  u32 *ctr = nullptr;     // [4] ticket, done, overflow count
   u64 *okey = nullptr;    // [WARM_OVF_CAP] overflow records
   double *ov = nullptr;
   i32 *oid = nullptr;
@@ -42,10 +46,15 @@ struct WarmArgs {
   i32 *iout;
   double *mout;
   WarmMeta rd, wr;
-  u64 *status;  // [nb * 3] look-back words
+  u64 *status;  // [nb * 5] look-back words
+  // per-launch counters, rings of 4 indexed by epoch: this launch's ticket
+  // counter and overflow count, the overflow count of the list it reads,
+  // and the slots launch epoch+2 will use (cleared by ticket 0)
+  u32 *tick, *tick_next2, *ovf_r, *ovf_w, *ovf_next2;
   u32 *err;     // [2] epoch of the first failing launch, failure kinds
   u32 epoch;
   double mq, totmass, omega2, h, pre, hd, hp, fscale, finv;
+  u64 *prof;    // null, or [nb * 8] phase timestamps (wall clock) per bucket
 };
 
 // failure kinds (err[1] bits)
@@ -55,7 +64,9 @@ constexpr u32 WARM_ERR_LOVF = 4;      // too many far movers into one bucket
 constexpr u32 WARM_ERR_STALL = 8;     // look-back gave up (bounded wait)
 
 int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
-                     bool harm, int mode);
-int warm_reset_launch(hipStream_t st, u32 *ctr_a, u32 *ctr_b, u32 *err);
+                     bool harm, int mode, int shape);
+// zero the ticket ring, the overflow-count ring except slot `keep` (4: none),
+// and the error word
+int warm_reset_launch(hipStream_t st, u32 *ring, u32 keep, u32 *err);
 
 }  // namespace wdy

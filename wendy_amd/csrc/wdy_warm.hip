@@ -4,59 +4,88 @@
 // value buckets whose boundaries ("splitters") are the sorted positions at
 // ranks k*T of two steps earlier.  The previous step left, for every source
 // bucket t, its sheets partitioned into contiguous segments by destination
-// bucket t + d - W (|d - W| <= W), plus one segment of far movers that are
+// bucket t + d - W (0 <= d <= 2W), plus one segment of far movers that are
 // also copied into an overflow list (see WarmMeta).
 //
-// Workgroup j (one per bucket, ticketed in launch order):
-//   1. gathers its incoming segments and overflow entries into LDS, applying
-//      the interval's first half drift when asked (wendy/wendy.c:318-323,398);
-//   2. publishes its sheet count and fixed-point mass total, then sorts the
-//      bucket in LDS by (position, sheet index): one counting pass over 4096
-//      position bins, then insertion fix-ups of the (tiny) bins;
-//   3. resolves its global rank offset and mass prefix with a decoupled
-//      look-back over integer values, so nothing depends on timing;
-//   4. computes force, kick and drift in sorted order (wendy/wendy.c:371-383,
+// Workgroup j (one per bucket, ticketed in launch order) runs a short chain
+// of dependent global-memory round trips, the rest being LDS work:
+//   1. takes its ticket (and reads the error word and the overflow count);
+//   2. reads its incoming segment table and the window of splitters it will
+//      partition with, together;
+//   3. gathers the bucket into LDS, all loads of a thread issued together,
+//      applying the interval's first half drift when asked
+//      (wendy/wendy.c:318-323,398), and publishes its sheet count and
+//      fixed-point mass total;
+//   4. sorts the bucket in LDS by (position, sheet index): one counting pass
+//      over position bins, then insertion fix-ups of the (tiny) bins;
+//   5. resolves its global rank offset and mass prefix with a decoupled
+//      look-back (one wave probing 64 predecessors at a time) over integer
+//      values, so nothing depends on timing;
+//   6. computes force, kick and drift in sorted order (wendy/wendy.c:371-383,
 //      324-333) and emits the splitters for two steps ahead;
-//   5. partitions the drifted sheets by next-step bucket (binary search in
-//      the window of splitters) and writes them to its output region
+//   7. partitions the drifted sheets by next-step bucket (binary search in
+//      the prefetched window) and writes them to its output region
 //      [offset, offset + count) with the segment table.
-// Mode 1 of the same kernel only does step 5, for an array already sorted
+// Mode 1 of the same kernel only does step 7, for an array already sorted
 // by the radix path (tile j = ranks [j*T, (j+1)*T)).
+//
+// Per-launch counters (tickets, overflow counts) live in rings of 4 indexed
+// by the launch epoch; ticket 0 of launch e clears the slots launch e+2 will
+// use, so no end-of-kernel bookkeeping is needed.
 //
 // Masses enter the prefix as 64-bit fixed point: m_fix = llrint(m * 2^k)
 // with 2^k * totmass <= 2^62, so bucket totals and prefixes are exact
 // integers whatever order they are added in; C = prefix * 2^-k rounds once.
 // Equal masses need no prefix: C = rank * m.
 //
-// Failure (a bucket over the LDS capacity, a full overflow list) records the
-// failing launch's epoch in err[0]; later launches exit before taking a
-// ticket, and the host redoes the interval from its untouched start state on
-// the radix path.
+// Failure (a bucket over capacity, a full overflow list, a look-back that
+// exceeds its bounded wait) records the failing launch's epoch in err[0];
+// later launches exit at once, and the host redoes the interval from its
+// untouched start state on the radix path.
+#include <cstdlib>
+#include <vector>
+
 #include "wdy_warm.h"
 
 namespace wdy {
 namespace {
 
-constexpr int WT = WARM_THREADS;
-constexpr int NWAVE = WT / 64;
-constexpr int NQ = WARM_NQ;
 constexpr int SMAX = 2 * WARM_WMAX + 1;
-constexpr int LOVF = 1024;  // overflow entries one bucket may take
+constexpr int LOVF = 256;  // overflow entries one bucket may take
 
-__device__ __forceinline__ u64 ld_acq(const u64 *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ u64 ld_rlx(const u64 *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel(u64 *p, u64 v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_rlx(u64 *p, u64 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ u32 ld_u32(const u32 *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+#define WARM_STAMP(k)                                              \
+  do {                                                             \
+    if (A.prof && t == 0) A.prof[8ull * j + (k)] = wall_clock64(); \
+  } while (0)
+
+// Look-back status of a bucket: 5 words, each tagged with the launch epoch
+// so that relaxed atomics suffice (no acquire/release, which on a multi-XCD
+// part cost L2 write-backs and invalidations):
+//   [0] epoch<<34 | flag<<32 | count   (flag 1 aggregate, 2 inclusive)
+//   [1],[2] aggregate mass, high / low 32 bits, each epoch<<34 | 1<<32 | half
+//   [3],[4] inclusive mass, likewise
+// A reader that sees the count word before the mass words retries.
+constexpr u64 STATUS_WORDS = 5;
+
+__device__ __forceinline__ void publish(u64 *st, u64 ep, u32 flag, u32 cnt,
+                                        u64 mass, bool with_mass) {
+  if (with_mass) {
+    u64 *mw = st + (flag == 2 ? 3 : 1);
+    const u64 tag = ep | (1ull << 32);
+    st_rlx(mw, tag | (mass >> 32));
+    st_rlx(mw + 1, tag | (mass & 0xffffffffull));
+  }
+  st_rlx(st, ep | ((u64)flag << 32) | cnt);
 }
 
 __device__ void set_err(u32 *err, u32 epoch, u32 kind) {
@@ -69,6 +98,7 @@ __device__ __forceinline__ u64 mass_fix(double m, double fscale) {
 }
 
 // Exclusive scan of one u64 per thread over the block; *total gets the sum.
+template <int NWAVE>
 __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
   const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   u64 x = v;
@@ -81,6 +111,7 @@ __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
   if (lane == 63) s_red[w] = x;
   __syncthreads();
   u64 wp = 0, tot = 0;
+#pragma unroll
   for (int i = 0; i < NWAVE; ++i) {
     if (i < (int)w) wp += s_red[i];
     tot += s_red[i];
@@ -89,12 +120,13 @@ __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
   return wp + x - v;
 }
 
-// Exclusive scan of s[0..n) (n <= WT) in place; returns the total.
+// Exclusive scan of s[0..n) (n <= threads) in place; returns the total.
+template <int NWAVE>
 __device__ u32 block_excl_u32_inplace(u32 *s, u32 n, u64 *s_red) {
   const u32 t = threadIdx.x;
   const u32 v = t < n ? s[t] : 0u;
   u64 tot;
-  const u64 e = block_excl_u64(v, s_red, &tot);
+  const u64 e = block_excl_u64<NWAVE>(v, s_red, &tot);
   __syncthreads();
   if (t < n) s[t] = (u32)e;
   __syncthreads();
@@ -105,11 +137,19 @@ __device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
   return ka < kb || (ka == kb && ia < ib);
 }
 
-template <bool EQ>
+// One tile shape: threads per workgroup and the most sheets a bucket holds.
+template <int WT_, int CAP_>
+struct Shape {
+  static constexpr int WT = WT_, CAP = CAP_, NQ = CAP_, PER = CAP_ / WT_;
+  static constexpr int NWAVE = WT_ / 64;
+  static_assert(CAP_ % WT_ == 0 && WT_ % 64 == 0 && CAP_ <= 65535, "shape");
+};
+
+template <class SH, bool EQ>
 struct Lds {
-  static constexpr int CAP = EQ ? WARM_CAP_EQ : WARM_CAP_NEQ;
+  static constexpr int CAP = SH::CAP;
   static constexpr size_t BYTES =
-      (size_t)CAP * (EQ ? 22 : 30) + (size_t)NQ * 4;
+      (size_t)CAP * (EQ ? 22 : 30) + (size_t)SH::NQ * 4;
   u64 *key;
   double *v;
   double *m;  // unequal masses only
@@ -127,15 +167,19 @@ struct Lds {
 };
 
 // s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
-template <bool EQ>
-__device__ void lds_bucket_sort(Lds<EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
-  constexpr int PER = (Lds<EQ>::CAP + WT - 1) / WT;
+template <class SH, bool EQ>
+__device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
+  constexpr int WT = SH::WT, PER = SH::PER, NQ = SH::NQ, NWAVE = SH::NWAVE;
   const u32 t = threadIdx.x;
   u64 kmin = ~0ull, kmax = 0ull;
-  for (u32 e = t; e < c; e += WT) {
-    const u64 k = s.key[e];
-    kmin = k < kmin ? k : kmin;
-    kmax = k > kmax ? k : kmax;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const u32 e = i * WT + t;
+    if (e < c) {
+      const u64 k = s.key[e];
+      kmin = k < kmin ? k : kmin;
+      kmax = k > kmax ? k : kmax;
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -147,10 +191,12 @@ __device__ void lds_bucket_sort(Lds<EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
     s_mm[2 * (t >> 6)] = kmin;
     s_mm[2 * (t >> 6) + 1] = kmax;
   }
-  for (u32 b = t; b < NQ; b += WT) s.bins[b] = 0;
+#pragma unroll
+  for (int i = 0; i < NQ / WT; ++i) s.bins[i * WT + t] = 0;
   __syncthreads();
   kmin = ~0ull;
   kmax = 0ull;
+#pragma unroll
   for (int i = 0; i < NWAVE; ++i) {
     kmin = s_mm[2 * i] < kmin ? s_mm[2 * i] : kmin;
     kmax = s_mm[2 * i + 1] > kmax ? s_mm[2 * i + 1] : kmax;
@@ -170,15 +216,16 @@ __device__ void lds_bucket_sort(Lds<EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
   u32 q[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const u32 e = t + i * WT;
+    const u32 e = i * WT + t;
     q[i] = 0;
     if (e < c) {
+      const u64 k = s.key[e];
       u32 b;
       if (dbl) {
-        const double qd = (decode_pos(s.key[e]) - xmin) * scale;
+        const double qd = (decode_pos(k) - xmin) * scale;
         b = qd < (double)NQ ? (u32)qd : (u32)(NQ - 1);
       } else {
-        b = (u32)((s.key[e] - kmin) >> kshift);
+        b = (u32)((k - kmin) >> kshift);
       }
       q[i] = b;
       atomicAdd(&s.bins[b], 1u);
@@ -186,7 +233,7 @@ __device__ void lds_bucket_sort(Lds<EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
   }
   __syncthreads();
   {
-    constexpr int BPT = NQ / WT;
+    constexpr int BPT = NQ / WT;  // bins t*BPT .. t*BPT + BPT - 1
     u32 loc[BPT];
     u64 sum = 0;
 #pragma unroll
@@ -194,7 +241,7 @@ __device__ void lds_bucket_sort(Lds<EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
       loc[i] = s.bins[t * BPT + i];
       sum += loc[i];
     }
-    u64 run = block_excl_u64(sum, s_red, nullptr);
+    u64 run = block_excl_u64<NWAVE>(sum, s_red, nullptr);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
@@ -205,26 +252,30 @@ __device__ void lds_bucket_sort(Lds<EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const u32 e = t + i * WT;
+    const u32 e = i * WT + t;
     if (e < c) s.ord[atomicAdd(&s.bins[q[i]], 1u)] = (unsigned short)e;
   }
   __syncthreads();
   // bins[b] is now the end of bin b; insertion-sort bins holding >= 2 sheets
-  for (u32 b = t; b < NQ; b += WT) {
-    const u32 lo = b == 0 ? 0u : s.bins[b - 1];
-    const u32 hi = s.bins[b];
-    for (u32 a = lo + 1; a < hi; ++a) {
-      const unsigned short ea = s.ord[a];
-      const u64 ka = s.key[ea];
-      const i32 ia = s.id[ea];
-      u32 p = a;
-      while (p > lo) {
-        const unsigned short eb = s.ord[p - 1];
-        if (!pair_lt(ka, ia, s.key[eb], s.id[eb])) break;
-        s.ord[p] = eb;
-        --p;
+  {
+    constexpr int BPT = NQ / WT;
+    u32 lo = t == 0 ? 0u : s.bins[t * BPT - 1];
+    for (int i = 0; i < BPT; ++i) {
+      const u32 hi = s.bins[t * BPT + i];
+      for (u32 a = lo + 1; a < hi; ++a) {
+        const unsigned short ea = s.ord[a];
+        const u64 ka = s.key[ea];
+        const i32 ia = s.id[ea];
+        u32 p = a;
+        while (p > lo) {
+          const unsigned short eb = s.ord[p - 1];
+          if (!pair_lt(ka, ia, s.key[eb], s.id[eb])) break;
+          s.ord[p] = eb;
+          --p;
+        }
+        s.ord[p] = ea;
       }
-      s.ord[p] = ea;
+      lo = hi;
     }
   }
   __syncthreads();
@@ -251,12 +302,11 @@ __device__ __forceinline__ u32 seg_pos(const u32 *segpre, const u32 *segsrc,
   return segsrc[lo] + (e - segpre[lo]);
 }
 
-template <bool EQ, bool HARM, int MODE>
-__global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
-  constexpr int CAP = Lds<EQ>::CAP;
-  constexpr int PER = (CAP + WT - 1) / WT;
+template <class SH, bool EQ, bool HARM, int MODE>
+__global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
+  constexpr int WT = SH::WT, PER = SH::PER, CAP = SH::CAP, NWAVE = SH::NWAVE;
   extern __shared__ u64 lds_dyn[];
-  Lds<EQ> s(lds_dyn);
+  Lds<SH, EQ> s(lds_dyn);
   __shared__ u64 s_spl[SMAX + 1];
   __shared__ u32 s_segsrc[SMAX];
   __shared__ u32 s_segpre[SMAX + 1];
@@ -268,23 +318,32 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   const u32 S = A.S, W = A.W, nb = A.nb;
-  {
-    const u32 e0 = ld_u32(A.err);
-    if (e0 != 0 && e0 < A.epoch) return;  // an earlier launch failed
+  // ---- 1. ticket, error word and overflow count: one round trip ------------
+  if (t == 0) {
+    s_u[0] = atomicAdd(A.tick, 1u);
+    s_u[1] = ld_u32(A.err);
+    s_u[3] = 0;  // overflow entries destined to this bucket
   }
-  if (t == 0) s_u[0] = atomicAdd(&A.wr.ctr[0], 1u);
+  const u32 novf_in = MODE == 0 ? ld_u32(A.ovf_r) : 0u;
   __syncthreads();
   const u32 j = s_u[0];
+  if (s_u[1] != 0 && s_u[1] < A.epoch) return;  // an earlier launch failed
   if (j >= nb) {  // tickets start at 0 each launch; never expected
     if (t == 0) set_err(A.err, A.epoch, WARM_ERR_STALL);
     return;
   }
+  if (j == 0 && t == 0) {  // clear the slots launch epoch+2 will use
+    *A.tick_next2 = 0;
+    *A.ovf_next2 = 0;
+  }
+  WARM_STAMP(0);
+  const u32 lo = j >= W ? j - W : 0u;
+  const u32 hi = j + W < nb ? j + W : nb - 1;
 
   u32 c = 0, off = 0;
-  bool ok = true;
 
   if (MODE == 0) {
-    // ---- 1. incoming segments and overflow entries --------------------------
+    // ---- 2. segment table and splitter window: one round trip ---------------
     if (t < S) {
       const int src = (int)j - (int)t + (int)W;
       u32 cnt = 0, start = 0;
@@ -295,48 +354,56 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
       s_segsrc[t] = start;
       s_segpre[t] = cnt;
     }
-    if (t == 0) s_u[1] = 0;
-    __syncthreads();
-    const u32 nseg = block_excl_u32_inplace(s_segpre, S, s_red);
-    u32 novf_all = ld_u32(A.rd.ctr + 2);
-    if (novf_all > WARM_OVF_CAP) novf_all = WARM_OVF_CAP;
+    for (u32 b = t; b <= hi - lo + 1; b += WT) s_spl[b] = A.rd.spl[lo + b];
+    u32 novf_all = novf_in > WARM_OVF_CAP ? WARM_OVF_CAP : novf_in;
     for (u32 q = t; q < novf_all; q += WT)
       if (A.rd.odst[q] == j) {
-        const u32 slot = atomicAdd(&s_u[1], 1u);
+        const u32 slot = atomicAdd(&s_u[3], 1u);
         if (slot < (u32)LOVF) s_oidx[slot] = q;
       }
     __syncthreads();
-    const u32 novf = s_u[1];
+    const u32 nseg = block_excl_u32_inplace<NWAVE>(s_segpre, S, s_red);
+    const u32 novf = s_u[3];
     c = nseg + novf;
-    ok = c <= (u32)CAP && novf <= (u32)LOVF;
+    const bool ok = c <= (u32)CAP && novf <= (u32)LOVF;
 
-    // ---- 2. gather, with the optional first half drift -----------------------
+    // ---- 3. gather: positions first, then every load of the thread --------
     u64 msum = 0;
     if (ok) {
-      for (u32 e = t; e < c; e += WT) {
-        u64 k;
-        double v, m = 0.0;
-        i32 id;
-        if (e < nseg) {
-          const u32 p = seg_pos(s_segpre, s_segsrc, S, e);
-          k = A.kin[p];
-          v = A.vin[p];
-          id = A.iin[p];
-          if (!EQ) m = A.min[p];
-        } else {
-          const u32 q = s_oidx[e - nseg];
-          k = A.rd.okey[q];
-          v = A.rd.ov[q];
-          id = A.rd.oid[q];
-          if (!EQ) m = A.rd.om[q];
+      u32 pos[PER];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 e = i * WT + t;
+        pos[i] = e < nseg ? seg_pos(s_segpre, s_segsrc, S, e)
+                          : (e < c ? s_oidx[e - nseg] : 0u);
+      }
+      u64 k[PER];
+      double v[PER], m[PER];
+      i32 id[PER];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 e = i * WT + t;
+        if (e < c) {
+          const bool seg = e < nseg;
+          k[i] = (seg ? A.kin : A.rd.okey)[pos[i]];
+          v[i] = (seg ? A.vin : A.rd.ov)[pos[i]];
+          id[i] = (seg ? A.iin : A.rd.oid)[pos[i]];
+          if (!EQ) m[i] = (seg ? A.min : A.rd.om)[pos[i]];
         }
-        if (A.pre != 0.0) k = encode_pos(decode_pos(k) + A.pre * v);
-        s.key[e] = k;
-        s.v[e] = v;
-        s.id[e] = id;
-        if (!EQ) {
-          s.m[e] = m;
-          msum += mass_fix(m, A.fscale);
+      }
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 e = i * WT + t;
+        if (e < c) {
+          u64 kk = k[i];
+          if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * v[i]);
+          s.key[e] = kk;
+          s.v[e] = v[i];
+          s.id[e] = id[i];
+          if (!EQ) {
+            s.m[e] = m[i];
+            msum += mass_fix(m[i], A.fscale);
+          }
         }
       }
     } else if (!EQ) {
@@ -347,113 +414,155 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
         if (A.rd.odst[q] == j) msum += mass_fix(A.rd.om[q], A.fscale);
     }
     u64 mtot = 0;
-    if (!EQ) (void)block_excl_u64(msum, s_red, &mtot);
-
-    // ---- 3. publish the aggregate --------------------------------------------
+    if (!EQ) (void)block_excl_u64<NWAVE>(msum, s_red, &mtot);
     const u64 ep = (u64)(A.epoch & 0x3FFFFFFFu) << 34;
     if (t == 0) {
-      u64 *st = A.status + 3ull * j;
-      if (j == 0) {
-        st_rlx(st + 2, mtot);
-        st_rel(st, ep | (2ull << 32) | c);
-      } else {
-        st_rlx(st + 1, mtot);
-        st_rel(st, ep | (1ull << 32) | c);
-      }
+      publish(A.status + STATUS_WORDS * j, ep, j == 0 ? 2u : 1u, c, mtot, !EQ);
       if (!ok)
         set_err(A.err, A.epoch,
                 novf > (u32)LOVF ? WARM_ERR_LOVF : WARM_ERR_CAPACITY);
     }
+    WARM_STAMP(1);
 
-    // ---- 4. local sort, while the predecessors publish ------------------------
-    if (ok) lds_bucket_sort<EQ>(s, c, s_red, s_mm);
+    // ---- 4. sort the bucket in LDS, while the predecessors publish ---------
+    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm);
+    else __syncthreads();
+    WARM_STAMP(2);
 
-    // ---- 5. look-back ------------------------------------------------------------
-    if (t == 0) {
+    // ---- 5. look-back, one wave probing 64 predecessors at a time ----------
+    if (w == 0) {
       u64 ecnt = 0, emass = 0;
-      s_u[2] = 0;
+      bool stalled = false;
       if (j > 0) {
-        int64_t pt = (int64_t)j - 1;
+        int64_t base = (int64_t)j - 1;  // lane l probes bucket base - l
         u32 spins = 0;
         while (true) {
-          const u64 *st = A.status + 3ull * (u64)pt;
-          const u64 wd = ld_acq(st);
-          const u32 flag = (u32)(wd >> 32) & 3u;
-          if ((u32)(wd >> 34) != (A.epoch & 0x3FFFFFFFu) || flag == 0) {
+          const int64_t pt = base - (int64_t)lane;
+          u64 wd = 0, mv = 0;
+          u32 flag = 2;  // buckets below 0: an inclusive prefix of zero
+          if (pt >= 0) {
+            const u64 *st = A.status + STATUS_WORDS * (u64)pt;
+            const u32 e30 = A.epoch & 0x3FFFFFFFu;
+            wd = ld_rlx(st);
+            flag = (u32)(wd >> 34) == e30 ? (u32)(wd >> 32) & 3u : 0u;
+            if (!EQ && flag != 0) {
+              const u64 *mw = st + (flag == 2 ? 3 : 1);
+              const u64 hw = ld_rlx(mw), lw = ld_rlx(mw + 1);
+              const bool good = (u32)(hw >> 32) == ((e30 << 2) | 1u) &&
+                                (u32)(lw >> 32) == ((e30 << 2) | 1u);
+              if (good) mv = ((hw & 0xffffffffull) << 32) | (lw & 0xffffffffull);
+              else flag = 0;  // mass not visible yet: retry
+            }
+          }
+          const u64 incm = __ballot(flag == 2);
+          const u64 zerom = __ballot(flag == 0);
+          const u32 first = incm ? (u32)__ffsll((long long)incm) - 1u : 64u;
+          const u64 upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+          if (zerom & upto) {
             // every predecessor publishes without waiting on anyone, so this
             // ends; the bound turns a broken invariant into a reported
             // failure instead of a hang
             if (++spins > (1u << 20)) {
-              set_err(A.err, A.epoch, WARM_ERR_STALL);
-              s_u[2] = 1;
+              stalled = true;
               break;
             }
             __builtin_amdgcn_s_sleep(1);
             continue;
           }
-          ecnt += (u32)wd;
-          if (!EQ) emass += ld_rlx(st + (flag == 2 ? 2 : 1));
-          if (flag == 2) break;
-          --pt;
+          u64 cc = 0, mm = 0;
+          if (lane <= first && pt >= 0) {
+            cc = (u32)wd;
+            mm = mv;
+          }
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            cc += __shfl_down(cc, o, 64);
+            mm += __shfl_down(mm, o, 64);
+          }
+          ecnt += __shfl(cc, 0, 64);
+          emass += __shfl(mm, 0, 64);
+          if (first < 64) break;
+          base -= 64;
         }
-        u64 *mine = A.status + 3ull * j;
-        st_rlx(mine + 2, emass + mtot);
-        st_rel(mine, ep | (2ull << 32) | (ecnt + c));
+        if (lane == 0) {
+          if (stalled) {
+            set_err(A.err, A.epoch, WARM_ERR_STALL);
+          } else {
+            publish(A.status + STATUS_WORDS * j, ep, 2u, (u32)(ecnt + c),
+                    emass + mtot, !EQ);
+          }
+        }
       }
-      s_u64[0] = ecnt;
-      s_u64[1] = emass;
+      if (lane == 0) {
+        s_u64[0] = ecnt;
+        s_u64[1] = emass;
+        s_u[2] = stalled ? 1u : 0u;
+      }
     }
     __syncthreads();
+    WARM_STAMP(3);
     off = (u32)s_u64[0];
     const u64 bfix = s_u64[1];
-    if (!ok || s_u[2]) goto done;  // nothing of this bucket is written
+    if (!ok || s_u[2]) return;  // nothing of this bucket is written
 
     // ---- 6. force, kick, drift in sorted order -----------------------------------
     {
-      const u32 r0 = t * PER;
+      const u32 r0 = t * PER;  // ranks r0 .. r0 + PER - 1, in order
       u64 run = 0;
       if (!EQ) {
         u64 part = 0;
+#pragma unroll
         for (int i = 0; i < PER; ++i)
           if (r0 + i < c) part += mass_fix(s.m[s.ord[r0 + i]], A.fscale);
-        run = bfix + block_excl_u64(part, s_red, nullptr);
+        run = bfix + block_excl_u64<NWAVE>(part, s_red, nullptr);
       }
+#pragma unroll
       for (int i = 0; i < PER; ++i) {
         const u32 r = r0 + i;
-        if (r >= c) break;
-        const u32 e = s.ord[r];
-        const u32 R = off + r;
-        const u64 kold = s.key[e];
-        double x = decode_pos(kold);
-        double F;
-        if (EQ) {
-          F = (A.totmass - 2.0 * ((double)R * A.mq)) - A.mq;
-        } else {
-          const double me = s.m[e];
-          F = (A.totmass - 2.0 * ((double)run * A.finv)) - me;
-          run += mass_fix(me, A.fscale);
+        if (r < c) {
+          const u32 e = s.ord[r];
+          const u32 R = off + r;
+          const u64 kold = s.key[e];
+          double x = decode_pos(kold);
+          double F;
+          if (EQ) {
+            F = (A.totmass - 2.0 * ((double)R * A.mq)) - A.mq;
+          } else {
+            const double me = s.m[e];
+            F = (A.totmass - 2.0 * ((double)run * A.finv)) - me;
+            run += mass_fix(me, A.fscale);
+          }
+          if (HARM) F = F - A.omega2 * x;
+          const double a = 0.0 + F;
+          const double v = s.v[e] + A.h * a;
+          x = x + A.hd * v;
+          s.v[e] = v;
+          s.key[e] = encode_pos(x);
+          if (R % A.T == 0) A.wr.spl[R / A.T] = R == 0 ? 0ull : kold;
         }
-        if (HARM) F = F - A.omega2 * x;
-        const double a = 0.0 + F;
-        const double v = s.v[e] + A.h * a;
-        x = x + A.hd * v;
-        s.v[e] = v;
-        s.key[e] = encode_pos(x);
-        if (R % A.T == 0) A.wr.spl[R / A.T] = R == 0 ? 0ull : kold;
       }
       if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
     }
     __syncthreads();
+    WARM_STAMP(4);
   } else {
     // MODE 1: bucket j is the sorted tile [j*T, min(n, (j+1)*T))
     off = j * A.T;
     c = min(A.T, A.n - off);
-    for (u32 e = t; e < c; e += WT) {
-      s.key[e] = A.kin[off + e];
-      s.v[e] = A.vin[off + e];
-      s.id[e] = A.iin[off + e];
-      if (!EQ) s.m[e] = A.min[off + e];
-      s.ord[e] = (unsigned short)e;
+    for (u32 b = t; b <= hi - lo + 1; b += WT) s_spl[b] = A.rd.spl[lo + b];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const u32 e = i * WT + t;
+      if (e < c) {
+        const u64 k = A.kin[off + e];
+        const double v = A.vin[off + e];
+        const i32 id = A.iin[off + e];
+        s.key[e] = k;
+        s.v[e] = v;
+        s.id[e] = id;
+        if (!EQ) s.m[e] = A.min[off + e];
+        s.ord[e] = (unsigned short)e;
+      }
     }
     if (t == 0) {
       A.wr.spl[j] = A.rd.spl[j];
@@ -464,16 +573,13 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
 
   // ---- 7. partition by next-step bucket and write out ---------------------------
   {
-    const u32 lo = j >= W ? j - W : 0u;
-    const u32 hi = j + W < nb ? j + W : nb - 1;
-    for (u32 b = t; b <= hi - lo + 1; b += WT) s_spl[b] = A.rd.spl[lo + b];
     u32 *wc = s.bins;  // [NWAVE][S + 1] per-wave segment counters
-    for (u32 b = t; b < NWAVE * (S + 1); b += WT) wc[b] = 0;
+    for (u32 b = t; b < (u32)NWAVE * (S + 1); b += WT) wc[b] = 0;
     __syncthreads();
-    u32 dd[PER], rk[PER];
+    u32 dd[PER], rr[PER];
 #pragma unroll
-    for (int row = 0; row < PER; ++row) {
-      const u32 r = row * WT + t;
+    for (int i = 0; i < PER; ++i) {
+      const u32 r = i * WT + t;
       const bool valid = r < c;
       u32 d = S;
       if (valid) {
@@ -491,7 +597,7 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
           d = lo + a + W - j;
         } else {
           const u32 dst = bucket_of(A.rd.spl, nb, kp);
-          const u32 q = atomicAdd(&A.wr.ctr[2], 1u);
+          const u32 q = atomicAdd(A.ovf_w, 1u);
           if (q < WARM_OVF_CAP) {
             A.wr.okey[q] = s.key[e];
             A.wr.ov[q] = s.v[e];
@@ -514,12 +620,13 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
       const u32 below_l = (u32)__popcll(peers & ((1ull << lane) - 1ull));
       const u32 old = valid ? wc[w * (S + 1) + d] : 0u;
       if (valid && below_l == 0) wc[w * (S + 1) + d] = old + (u32)__popcll(peers);
-      dd[row] = d;
-      rk[row] = old + below_l;
+      dd[i] = d;
+      rr[i] = old + below_l;
     }
     __syncthreads();
     if (t <= S) {  // thread t owns segment t: per-wave prefixes and total
       u32 tot = 0;
+#pragma unroll
       for (int i = 0; i < NWAVE; ++i) {
         const u32 x = wc[i * (S + 1) + t];
         wc[i * (S + 1) + t] = tot;
@@ -528,7 +635,7 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
       s_segpre[t] = tot;
     }
     __syncthreads();
-    block_excl_u32_inplace(s_segpre, S + 1, s_red);
+    block_excl_u32_inplace<NWAVE>(s_segpre, S + 1, s_red);
     if (t <= S) {
       const u32 nxt = t < S ? s_segpre[t + 1] : c;
       A.wr.segoff[j * (S + 1) + t] = s_segpre[t];
@@ -536,11 +643,11 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
     }
     if (t == 0) A.wr.boff[j] = off;
 #pragma unroll
-    for (int row = 0; row < PER; ++row) {
-      const u32 r = row * WT + t;
+    for (int i = 0; i < PER; ++i) {
+      const u32 r = i * WT + t;
       if (r < c) {
         const u32 e = s.ord[r];
-        const u32 p = off + s_segpre[dd[row]] + wc[w * (S + 1) + dd[row]] + rk[row];
+        const u32 p = off + s_segpre[dd[i]] + wc[w * (S + 1) + dd[i]] + rr[i];
         A.kout[p] = s.key[e];
         A.vout[p] = s.v[e];
         A.iout[p] = s.id[e];
@@ -548,70 +655,70 @@ __global__ __launch_bounds__(WT) void k_warm_step(WarmArgs A) {
       }
     }
   }
-
-done:
-  // ---- 8. the last workgroup resets the counters the next launches reuse ------
-  __syncthreads();
-  if (t == 0) {
-    __threadfence();
-    const u32 d = atomicAdd(&A.wr.ctr[1], 1u);
-    if (d == nb - 1) {
-      if (MODE == 0) {
-        A.rd.ctr[0] = 0;
-        A.rd.ctr[2] = 0;
-      }
-      A.wr.ctr[1] = 0;
-    }
-  }
+  WARM_STAMP(5);
 }
 
-__global__ void k_warm_reset(u32 *ctr_a, u32 *ctr_b, u32 *err) {
+__global__ void k_warm_reset(u32 *ring, u32 keep, u32 *err) {
   const u32 t = threadIdx.x;
-  if (t < 4) {
-    ctr_a[t] = 0;
-    ctr_b[t] = 0;
-  }
+  // ring[0..3] tickets, ring[4..7] overflow counts; slot `keep` of the
+  // overflow counts still describes the current state's overflow list
+  if (t < 4) ring[t] = 0;
+  if (t >= 4 && t < 8 && t - 4 != keep) ring[t] = 0;
   if (t < 2) err[t] = 0;
 }
 
-template <bool EQ, bool HARM, int MODE>
+template <class SH, bool EQ, bool HARM, int MODE>
 int launch_one(hipStream_t st, const WarmArgs &a) {
-  auto fn = k_warm_step<EQ, HARM, MODE>;
-  const size_t lds = Lds<EQ>::BYTES;
-  static bool attr_set = false;
-  if (!attr_set) {
+  auto fn = k_warm_step<SH, EQ, HARM, MODE>;
+  const size_t lds = Lds<SH, EQ>::BYTES;
+  if (lds > 65536)
     WDY_HIP(hipFuncSetAttribute((const void *)fn,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-    attr_set = true;
-  }
-  fn<<<a.nb, WT, lds, st>>>(a);
+  fn<<<a.nb, SH::WT, lds, st>>>(a);
   WDY_CHECK_LAUNCH();
   return WDY_OK;
+}
+
+template <class SH>
+int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
+                 int mode) {
+  if (mode == 0) {
+    if (eq)
+      return harm ? launch_one<SH, true, true, 0>(st, a)
+                  : launch_one<SH, true, false, 0>(st, a);
+    return harm ? launch_one<SH, false, true, 0>(st, a)
+                : launch_one<SH, false, false, 0>(st, a);
+  }
+  if (eq)
+    return harm ? launch_one<SH, true, true, 1>(st, a)
+                : launch_one<SH, true, false, 1>(st, a);
+  return harm ? launch_one<SH, false, true, 1>(st, a)
+              : launch_one<SH, false, false, 1>(st, a);
 }
 
 }  // namespace
 
+// Tile shapes (threads per workgroup, bucket capacity in sheets).
+static const WarmShape kShapes[WARM_NSHAPES] = {
+    {1024, 4096}, {512, 2048}, {256, 1024}, {512, 1024}};
+
+const WarmShape &warm_shape(int v) { return kShapes[v]; }
+
 int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
-                     bool harm, int mode) {
-  if (mode == 0) {
-    if (equal_mass)
-      return harm ? launch_one<true, true, 0>(st, a) : launch_one<true, false, 0>(st, a);
-    return harm ? launch_one<false, true, 0>(st, a) : launch_one<false, false, 0>(st, a);
+                     bool harm, int mode, int shape) {
+  switch (shape) {
+    case 0: return launch_shape<Shape<1024, 4096>>(st, a, equal_mass, harm, mode);
+    case 1: return launch_shape<Shape<512, 2048>>(st, a, equal_mass, harm, mode);
+    case 2: return launch_shape<Shape<256, 1024>>(st, a, equal_mass, harm, mode);
+    default: return launch_shape<Shape<512, 1024>>(st, a, equal_mass, harm, mode);
   }
-  if (equal_mass)
-    return harm ? launch_one<true, true, 1>(st, a) : launch_one<true, false, 1>(st, a);
-  return harm ? launch_one<false, true, 1>(st, a) : launch_one<false, false, 1>(st, a);
 }
 
-int warm_reset_launch(hipStream_t st, u32 *ctr_a, u32 *ctr_b, u32 *err) {
-  k_warm_reset<<<1, 64, 0, st>>>(ctr_a, ctr_b, err);
+int warm_reset_launch(hipStream_t st, u32 *ring, u32 keep, u32 *err) {
+  k_warm_reset<<<1, 64, 0, st>>>(ring, keep, err);
   WDY_CHECK_LAUNCH();
   return WDY_OK;
-}
-
-u32 warm_capacity(bool equal_mass) {
-  return equal_mass ? (u32)WARM_CAP_EQ : (u32)WARM_CAP_NEQ;
 }
 
 }  // namespace wdy
@@ -623,6 +730,8 @@ namespace wdy {
 
 struct WarmCtx {
   u32 T = 0, W = 0, S = 0, nb = 0, cap = 0;
+  int shape = WARM_SHAPE_DEFAULT;
+  u32 *ring = nullptr;   // [8] ticket ring, overflow-count ring
   size_t seg_alloc = 0, nb_alloc = 0;  // allocated entries
   WarmMeta meta[3];
   u64 *status = nullptr;
@@ -633,10 +742,16 @@ struct WarmCtx {
   double hp = 0.0;       // half drift the partition anticipates
   int rd = 0;
   double fscale = 1.0, finv = 1.0;
+  // optional per-phase profile (environment variable WDY_PROFILE_WARM=1)
+  u64 *prof = nullptr;
+  size_t prof_alloc = 0;
+  double prof_us[6] = {};  // gather, sort, look-back, force, write, span
+  int64_t prof_n = 0;
+  double clk_khz = 100000.0;
 };
 
 static void meta_free(WarmMeta &m) {
-  void *ps[] = {m.segcnt, m.segoff, m.boff, m.spl, m.ctr,
+  void *ps[] = {m.segcnt, m.segoff, m.boff, m.spl,
                 m.okey, m.ov, m.oid, m.om, m.odst};
   for (void *p : ps)
     if (p) (void)hipFree(p);
@@ -648,8 +763,6 @@ static int meta_alloc(WarmMeta &m, size_t nseg, size_t nbp1) {
   WDY_HIP(hipMalloc(&m.segoff, nseg * 4));
   WDY_HIP(hipMalloc(&m.boff, nbp1 * 4));
   WDY_HIP(hipMalloc(&m.spl, nbp1 * 8));
-  WDY_HIP(hipMalloc(&m.ctr, 16));
-  WDY_HIP(hipMemset(m.ctr, 0, 16));
   WDY_HIP(hipMalloc(&m.okey, (size_t)WARM_OVF_CAP * 8));
   WDY_HIP(hipMalloc(&m.ov, (size_t)WARM_OVF_CAP * 8));
   WDY_HIP(hipMalloc(&m.oid, (size_t)WARM_OVF_CAP * 4));
@@ -661,21 +774,38 @@ static int meta_alloc(WarmMeta &m, size_t nseg, size_t nbp1) {
 void warm_free(wdy_sim *s) {
   WarmCtx *w = s->warm;
   if (!w) return;
+  if (w->prof) {
+    if (w->prof_n > 0) {
+      const double k = 1.0 / (double)w->prof_n;
+      fprintf(stderr,
+              "wendy_amd warm-step profile (n=%lld, %lld launches, mean us "
+              "per bucket): gather %.2f sort %.2f look-back %.2f force %.2f "
+              "partition+write %.2f | launch span %.1f\n",
+              (long long)s->n, (long long)w->prof_n, w->prof_us[0] * k,
+              w->prof_us[1] * k, w->prof_us[2] * k, w->prof_us[3] * k,
+              w->prof_us[4] * k, w->prof_us[5] * k);
+    }
+    (void)hipFree(w->prof);
+  }
   for (auto &m : w->meta) meta_free(m);
   if (w->status) (void)hipFree(w->status);
   if (w->err) (void)hipFree(w->err);
+  if (w->ring) (void)hipFree(w->ring);
   if (w->h_err) (void)hipHostFree(w->h_err);
   delete w;
   s->warm = nullptr;
 }
 
-// (Re)sizes the metadata for bucket target T and window W.
-int warm_configure(wdy_sim *s, u32 T, u32 W) {
+// (Re)sizes the metadata for tile shape `shape` (-1: keep), bucket target T
+// (0: 3/4 of the shape's capacity) and window W.
+int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
   if (!s->warm) {
     s->warm = new (std::nothrow) WarmCtx();
     if (!s->warm) return fail(WDY_ENOMEM, "warm context allocation failed");
     WDY_HIP(hipMalloc(&s->warm->err, 8));
     WDY_HIP(hipMemset(s->warm->err, 0, 8));
+    WDY_HIP(hipMalloc(&s->warm->ring, 32));
+    WDY_HIP(hipMemset(s->warm->ring, 0, 32));
     WDY_HIP(hipHostMalloc((void **)&s->warm->h_err, 8, hipHostMallocDefault));
     // fixed-point scale: 2^k * totmass <= 2^62
     int e = 0;
@@ -685,7 +815,10 @@ int warm_configure(wdy_sim *s, u32 T, u32 W) {
   }
   WarmCtx &w = *s->warm;
   const u32 n = (u32)s->n;
-  w.cap = s->equal_mass ? (u32)WARM_CAP_EQ : (u32)WARM_CAP_NEQ;
+  if (shape >= 0) w.shape = shape;
+  w.cap = (u32)warm_shape(w.shape).cap;
+  if (T == 0) T = w.cap / 4 * 3;
+  if (T > w.cap) T = w.cap;
   w.T = T;
   w.W = W;
   w.S = 2 * W + 1;
@@ -697,13 +830,44 @@ int warm_configure(wdy_sim *s, u32 T, u32 W) {
     w.status = nullptr;
     for (auto &m : w.meta)
       if (int rc = meta_alloc(m, nseg, nbp1)) return rc;
-    WDY_HIP(hipMalloc(&w.status, nbp1 * 3 * 8));
-    WDY_HIP(hipMemset(w.status, 0, nbp1 * 3 * 8));
+    WDY_HIP(hipMalloc(&w.status, nbp1 * 5 * 8));
+    WDY_HIP(hipMemset(w.status, 0, nbp1 * 5 * 8));
     w.seg_alloc = nseg;
     w.nb_alloc = nbp1;
   }
+  const char *pe = getenv("WDY_PROFILE_WARM");
+  if (pe && pe[0] == '1' && nbp1 * 8 > w.prof_alloc) {
+    if (w.prof) (void)hipFree(w.prof);
+    WDY_HIP(hipMalloc(&w.prof, nbp1 * 8 * sizeof(u64)));
+    w.prof_alloc = nbp1 * 8;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                              s->device) == hipSuccess && khz > 0)
+      w.clk_khz = khz;
+  }
   w.ready = false;
   return WDY_OK;
+}
+
+// Folds the phase stamps of the last warm launch into the running means.
+static void prof_fold(wdy_sim *s) {
+  WarmCtx &w = *s->warm;
+  std::vector<u64> p((size_t)w.nb * 8);
+  if (hipMemcpy(p.data(), w.prof, p.size() * 8, hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return;
+  const double us = 1000.0 / w.clk_khz;
+  u64 t0 = ~0ull, t5 = 0;
+  double d[5] = {};
+  for (u32 j = 0; j < w.nb; ++j) {
+    const u64 *q = &p[8ull * j];
+    for (int k = 0; k < 5; ++k) d[k] += (double)(q[k + 1] - q[k]) * us;
+    t0 = q[0] < t0 ? q[0] : t0;
+    t5 = q[5] > t5 ? q[5] : t5;
+  }
+  for (int k = 0; k < 5; ++k) w.prof_us[k] += d[k] / w.nb;
+  w.prof_us[5] += (double)(t5 - t0) * us;
+  w.prof_n += 1;
 }
 
 u64 *warm_cold_splitters(wdy_sim *s, u32 *T, u32 *nb) {
@@ -729,7 +893,17 @@ static WarmArgs base_args(wdy_sim *s) {
   a.fscale = w.fscale;
   a.finv = w.finv;
   a.h = a.pre = a.hd = a.hp = 0.0;
+  a.prof = nullptr;
   return a;
+}
+
+static void set_epoch(WarmArgs &a, WarmCtx &w, u32 e) {
+  a.epoch = e;
+  a.tick = w.ring + (e & 3);
+  a.tick_next2 = w.ring + ((e + 2) & 3);
+  a.ovf_r = w.ring + 4 + ((e + 3) & 3);
+  a.ovf_w = w.ring + 4 + (e & 3);
+  a.ovf_next2 = w.ring + 4 + ((e + 2) & 3);
 }
 
 static void set_state(WarmArgs &a, wdy_sim *s, int in, int out) {
@@ -750,17 +924,17 @@ int warm_partition_after_cold(wdy_sim *s, double hp) {
   WarmCtx &w = *s->warm;
   if (w.T > w.cap) return fail(WDY_EINVAL, "bucket target above capacity");
   const int x = (w.rd + 1) % 3, y = (w.rd + 2) % 3;
-  if (int rc = warm_reset_launch(s->stream, w.meta[x].ctr, w.meta[y].ctr, w.err))
-    return rc;
+  if (int rc = warm_reset_launch(s->stream, w.ring, 4u, w.err)) return rc;
   const int in = s->cur, out = (s->cur + 1) % 3;
   WarmArgs a = base_args(s);
   set_state(a, s, in, out);
   a.rd = w.meta[x];
   a.wr = w.meta[y];
-  a.epoch = ++w.epoch;
+  set_epoch(a, w, ++w.epoch);
   a.hp = hp;
   if (s->timing) timer_begin(s->timer, s->stream, KC_WARM);
-  int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0, 1);
+  int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0, 1,
+                            w.shape);
   if (s->timing) timer_end(s->timer, s->stream, KC_WARM);
   if (rc) return rc;
   s->cur = out;
@@ -768,6 +942,10 @@ int warm_partition_after_cold(wdy_sim *s, double hp) {
   w.ready = true;
   w.hp = hp;
   return WDY_OK;
+}
+
+int warm_current_shape(wdy_sim *s) {
+  return s->warm ? s->warm->shape : WARM_SHAPE_DEFAULT;
 }
 
 bool warm_ready_for(wdy_sim *s, double h) {
@@ -786,8 +964,9 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     if (i != start) so[ns++] = i;
     if (i != mstart) mo[nm++] = i;
   }
-  if (int rc = warm_reset_launch(s->stream, w.meta[mo[0]].ctr,
-                                 w.meta[mo[1]].ctr, w.err))
+  // keep the overflow count of the list the first launch reads (written by
+  // the launch with the current epoch)
+  if (int rc = warm_reset_launch(s->stream, w.ring, w.epoch & 3u, w.err))
     return rc;
   int in = start, rdm = mstart;
   for (int k = 0; k < nleap; ++k) {
@@ -796,13 +975,15 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     set_state(a, s, in, out);
     a.rd = w.meta[rdm];
     a.wr = w.meta[wrm];
-    a.epoch = ++w.epoch;
+    set_epoch(a, w, ++w.epoch);
     a.h = h;
     a.pre = k == 0 ? h / 2. : 0.0;
     a.hd = k == nleap - 1 ? h / 2. : h;
     a.hp = k == nleap - 1 ? h / 2. : 0.0;
+    a.prof = w.prof;
     if (s->timing) timer_begin(s->timer, s->stream, KC_WARM);
-    int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0, 0);
+    int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0, 0,
+                              w.shape);
     if (s->timing) timer_end(s->timer, s->stream, KC_WARM);
     if (rc) return rc;
     in = out;
@@ -810,6 +991,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
   }
   WDY_HIP(hipMemcpyAsync(w.h_err, w.err, 8, hipMemcpyDeviceToHost, s->stream));
   WDY_HIP(hipStreamSynchronize(s->stream));
+  if (w.prof && w.h_err[0] == 0) prof_fold(s);
   if (w.h_err[0] != 0) {
     const u32 kind = w.h_err[1];
     *failed = true;
@@ -828,7 +1010,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
       warm_free(s);
       return WDY_OK;
     }
-    if (int rc = warm_configure(s, T, W)) return rc;
+    if (int rc = warm_configure(s, -1, T, W)) return rc;
     return WDY_OK;
   }
   s->cur = in;

@@ -80,10 +80,11 @@ class Sim:
                                          ctypes.byref(e)))
         return e.value
 
-    def tune(self, bucket_target, window):
-        """Warm-start bucket target and window (large path); applies from the
-        next output interval."""
-        _lib.check(_lib.lib().wdy_tune(self._h, int(bucket_target), int(window)))
+    def tune(self, bucket_target=0, window=8, shape=-1):
+        """Warm-start tile shape, bucket target and window (large path);
+        applies from the next output interval (see wdy_tune)."""
+        _lib.check(_lib.lib().wdy_tune(self._h, int(shape), int(bucket_target),
+                                       int(window)))
 
     def path_stats(self):
         """(warm-start steps, radix steps, redone intervals) so far."""
