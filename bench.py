@@ -54,6 +54,93 @@ def workload(n, seed=0):
     return x, v, m
 
 
+def disk(n, rs, zh=1., sigma=1., lognormal=None):
+    """sech^2 disk of n sheets from RandomState rs: x = 2 zh atanh(2u-1),
+    v = sigma N(0,1), masses equal 1/n or lognormal(0, s) normalised to 1."""
+    u = rs.uniform(size=n)
+    bad = u == 0.
+    while numpy.any(bad):
+        u[bad] = rs.uniform(size=int(bad.sum()))
+        bad = u == 0.
+    x = 2. * zh * numpy.arctanh(2. * u - 1.)
+    v = sigma * rs.normal(size=n)
+    if lognormal is None:
+        m = numpy.full(n, 1. / n)
+    else:
+        m = rs.lognormal(0., lognormal, size=n)
+        m /= m.sum()
+    return x, v, m
+
+
+def ensemble_workload(nsims, seed=1, lo=8, hi=16):
+    """The mixed-size ensemble: N_k log-uniform in [2^lo, 2^hi], equal masses
+    1/N_k, z_h ~ U(0.5, 2), sigma = sqrt(z_h), omega_k ~ U(0, 0.2)."""
+    rs = numpy.random.RandomState(seed)
+    ns = numpy.floor(2. ** rs.uniform(lo, hi, nsims)).astype(int)
+    zhs = rs.uniform(0.5, 2., nsims)
+    oms = rs.uniform(0., 0.2, nsims)
+    xs, vs, ms = [], [], []
+    for n, zh in zip(ns, zhs):
+        x, v, m = disk(int(n), rs, zh, numpy.sqrt(zh))
+        xs.append(x)
+        vs.append(v)
+        ms.append(m)
+    return xs, vs, ms, [float(o) for o in oms]
+
+
+def bench_ensemble(args, log):
+    """Batched ensemble vs looping the plain GPU path over the same disks.
+
+    Batched: all simulations advance together, one launch per output
+    interval; the timed region is ``ens_steps`` intervals after one warm-up.
+    Plain: a sample of the same simulations (the first ``ens_sample``; sizes
+    are i.i.d.) each run alone through wendy_amd.Sim, one after the other,
+    with the same warm-up and step count; its rate is the sample's sheet-steps
+    over the sample's summed wall time."""
+    import wendy_amd
+    nleap, dt = 10, 0.01
+    h = dt / nleap
+    K = args.ens_steps
+    xs, vs, ms, oms = ensemble_workload(args.ens_sims)
+    ntot = int(sum(len(x) for x in xs))
+    log('ensemble: %d sims, %d sheets' % (len(xs), ntot))
+    b = wendy_amd.Batch(xs, vs, ms, oms)
+    b.advance(h, nleap)
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        b.advance(h, nleap)
+    b.sync()
+    tb = time.perf_counter() - t0
+    b.close()
+    rate_b = ntot * nleap * K / tb
+    log('ensemble: batched %.3e sheet-steps/s (%.2f ms/output)'
+        % (rate_b, 1e3 * tb / K))
+    S = min(args.ens_sample, len(xs))
+    work, tp = 0, 0.
+    for j in range(S):
+        m = ms[j]
+        sim = wendy_amd.Sim(xs[j], vs[j], m, numpy.sum(m), oms[j] ** 2.)
+        sim.advance(h, nleap)
+        sim.read()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            sim.advance(h, nleap)
+        sim.read()
+        tp += time.perf_counter() - t0
+        sim.close()
+        work += len(m) * nleap * K
+    rate_p = work / tp
+    log('ensemble: plain loop %.3e sheet-steps/s over %d sims' % (rate_p, S))
+    return {'workload': ('%d sech^2 disks, N log-uniform [2^8, 2^16], equal '
+                         'masses 1/N, z_h~U(0.5,2), sigma=sqrt(z_h), '
+                         'omega~U(0,0.2), dt=0.01, nleap=10' % len(xs)),
+            'sims': len(xs), 'sheets': ntot, 'steps': K,
+            'batched_value': rate_b, 'batched_ms_per_step': 1e3 * tb / K,
+            'plain_loop_value': rate_p, 'plain_loop_sample_sims': S,
+            'speedup': rate_b / rate_p, 'unit': 'sheet-steps/s'}
+
+
 def cpu_baseline(x, v, m, dt, nleap, omega, threads, budget_s=25.):
     """Time the reference C (sort='parallel') on a bounded sample.
 
@@ -100,6 +187,10 @@ def main():
     ap.add_argument('--omega', type=float, default=0.1)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=16)
+    ap.add_argument('--no-ensemble', action='store_true')
+    ap.add_argument('--ens-sims', type=int, default=4096)
+    ap.add_argument('--ens-steps', type=int, default=10)
+    ap.add_argument('--ens-sample', type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; --gpus %d is not '
@@ -171,6 +262,8 @@ def main():
                 'step_frac_B236': rate * 236 / HBM_PEAK,
                 'step_frac_B40': rate * 40 / HBM_PEAK}
 
+    ens = None if args.no_ensemble else bench_ensemble(args, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log('bench: cpu baseline ...')
@@ -193,6 +286,7 @@ def main():
         'kernel_ms': {str(c): kt[c][0] for c in kt},
         'roofline': roofline,
         'cpu_baseline': cpu,
+        'ensemble': ens,
     }
     print(json.dumps(out))
 

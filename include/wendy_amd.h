@@ -153,6 +153,28 @@ int wdy_potential_arrays(int64_t ny, const double *y, int64_t n,
                          const double *x, const double *m, double twopiG,
                          double omega2, double *out);
 
+/* ---- batched ensemble ------------------------------------------------------
+ * Many independent simulations (uneven n) advanced together: one kernel
+ * launch per output interval for all of them.  No reference counterpart
+ * (the reference runs one simulation per generator); each simulation's
+ * arithmetic is that of wdy_create/wdy_advance.
+ *   offsets   nsims + 1 increasing values, offsets[0] = 0: simulation k owns
+ *             sheets [offsets[k], offsets[k+1]) of x, v, m
+ *   m         masses already multiplied by twopiG (>= 0)
+ *   totmass   per simulation, numpy.sum of its m
+ *   omega2    per simulation, omega**2 or negative for none */
+typedef struct wdy_batch wdy_batch;
+int wdy_batch_create(int64_t nsims, const int64_t *offsets, const double *x,
+                     const double *v, const double *m, const double *totmass,
+                     const double *omega2, wdy_batch **out);
+/* One output interval of every simulation (same h and nleap for all). */
+int wdy_batch_advance(wdy_batch *batch, double h, int nleap);
+/* De-sorted x, v of all simulations, concatenated as at creation; blocks.
+ * seconds_out: device time of the last wdy_batch_advance. */
+int wdy_batch_read(wdy_batch *batch, double *x_out, double *v_out,
+                   double *seconds_out);
+int wdy_batch_destroy(wdy_batch *batch);
+
 /* ---- reference-signature compatibility entry point ----------------------
  * Same name, arguments and meaning as the reference's
  * _wendy_nbody_approx_onestep (wendy/wendy.c:385-393, wendy/wendy.h:29-34),

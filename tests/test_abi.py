@@ -130,3 +130,29 @@ def test_compat_symbol_rejects_ext_force_and_bad_sort():
     err.value = 0
     f(*args(None, 7))
     assert err.value == _lib.WDY_EINVAL
+
+
+def test_batch_create_rejects_bad_arguments_before_touching_device():
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    y = numpy.array([0., 1., 2., 3.])
+    one = numpy.array([1., 1.])
+    om = numpy.array([-1., -1.])
+    ok = numpy.array([0, 2, 4], dtype=numpy.int64)
+    for off in ([1, 2, 4], [0, 2, 2], [0, 3, 2]):
+        rc = lib.wdy_batch_create(2, numpy.array(off, dtype=numpy.int64),
+                                  y, y, y, one, om, ctypes.byref(h))
+        assert rc == _lib.WDY_EINVAL, off
+    bad = y.copy()
+    bad[3] = numpy.inf
+    assert lib.wdy_batch_create(2, ok, bad, y, y, one, om,
+                                ctypes.byref(h)) == _lib.WDY_EINVAL
+    neg = -y
+    neg[0] = -1.
+    assert lib.wdy_batch_create(2, ok, y, y, neg, one, om,
+                                ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert lib.wdy_batch_create(2, ok, y, y, y, numpy.array([1., numpy.nan]),
+                                om, ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert lib.wdy_batch_create(0, ok, y, y, y, one, om,
+                                ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert h.value is None

@@ -35,6 +35,8 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
+            'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
+            'wdy_batch_destroy',
             '_wendy_nbody_approx_onestep')
 
 _F64 = ndpointer(dtype=numpy.float64, flags=('C_CONTIGUOUS',))
@@ -92,6 +94,16 @@ def _declare(lib):
     lib.wdy_momentum_arrays.restype = ctypes.c_int
     lib.wdy_momentum_arrays.argtypes = [ctypes.c_int64, _F64, _F64,
                                         _c_double_p]
+    _I64 = ndpointer(dtype=numpy.int64, flags=('C_CONTIGUOUS',))
+    lib.wdy_batch_create.restype = ctypes.c_int
+    lib.wdy_batch_create.argtypes = [ctypes.c_int64, _I64, _F64, _F64, _F64,
+                                     _F64, _F64, ctypes.POINTER(S)]
+    lib.wdy_batch_advance.restype = ctypes.c_int
+    lib.wdy_batch_advance.argtypes = [S, ctypes.c_double, ctypes.c_int]
+    lib.wdy_batch_read.restype = ctypes.c_int
+    lib.wdy_batch_read.argtypes = [S, S, S, _c_double_p]
+    lib.wdy_batch_destroy.restype = ctypes.c_int
+    lib.wdy_batch_destroy.argtypes = [S]
     lib.wdy_potential_arrays.restype = ctypes.c_int
     lib.wdy_potential_arrays.argtypes = [ctypes.c_int64, _F64, ctypes.c_int64,
                                          _F64, _F64, ctypes.c_double,

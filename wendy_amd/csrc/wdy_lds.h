@@ -1,0 +1,203 @@
+// LDS building blocks shared by the warm-start step and the batched
+// ensemble kernel: block scans, the (position, sheet index) order, and the
+// in-LDS bucket sort (one counting pass over position bins plus insertion
+// fix-ups of the bins).
+#pragma once
+
+#include "wdy_internal.h"
+
+namespace wdy {
+
+__device__ __forceinline__ u64 mass_fix(double m, double fscale) {
+  return (u64)llrint(m * fscale);
+}
+
+// Exclusive scan of one u64 per thread over the block; *total gets the sum.
+template <int NWAVE>
+__device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
+  const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  u64 x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u64 y = __shfl_up(x, o, 64);
+    if (lane >= (u32)o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) s_red[w] = x;
+  __syncthreads();
+  u64 wp = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NWAVE; ++i) {
+    if (i < (int)w) wp += s_red[i];
+    tot += s_red[i];
+  }
+  if (total) *total = tot;
+  return wp + x - v;
+}
+
+// Exclusive scan of s[0..n) (n <= threads) in place; returns the total.
+template <int NWAVE>
+__device__ u32 block_excl_u32_inplace(u32 *s, u32 n, u64 *s_red) {
+  const u32 t = threadIdx.x;
+  const u32 v = t < n ? s[t] : 0u;
+  u64 tot;
+  const u64 e = block_excl_u64<NWAVE>(v, s_red, &tot);
+  __syncthreads();
+  if (t < n) s[t] = (u32)e;
+  __syncthreads();
+  return (u32)tot;
+}
+
+__device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+// One tile shape: threads per workgroup and the most sheets a bucket holds.
+template <int WT_, int CAP_>
+struct Shape {
+  static constexpr int WT = WT_, CAP = CAP_, NQ = CAP_, PER = CAP_ / WT_;
+  static constexpr int NWAVE = WT_ / 64;
+  static_assert(CAP_ % WT_ == 0 && WT_ % 64 == 0 && CAP_ <= 65535, "shape");
+};
+
+template <class SH, bool EQ>
+struct Lds {
+  static constexpr int CAP = SH::CAP;
+  static constexpr size_t BYTES =
+      (size_t)CAP * (EQ ? 22 : 30) + (size_t)SH::NQ * 4;
+  u64 *key;
+  double *v;
+  double *m;  // unequal masses only
+  i32 *id;
+  unsigned short *ord;
+  u32 *bins;
+  __device__ explicit Lds(u64 *base) {
+    key = base;
+    v = (double *)(key + CAP);
+    m = v + CAP;
+    id = (i32 *)(EQ ? (v + CAP) : (m + CAP));
+    ord = (unsigned short *)(id + CAP);
+    bins = (u32 *)(ord + CAP);
+  }
+};
+
+// s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
+template <class SH, bool EQ>
+__device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
+                                u64 *prof) {
+  constexpr int WT = SH::WT, PER = SH::PER, NQ = SH::NQ, NWAVE = SH::NWAVE;
+  const u32 t = threadIdx.x;
+  u64 kmin = ~0ull, kmax = 0ull;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const u32 e = i * WT + t;
+    if (e < c) {
+      const u64 k = s.key[e];
+      kmin = k < kmin ? k : kmin;
+      kmax = k > kmax ? k : kmax;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const u64 a = __shfl_down(kmin, o, 64), b = __shfl_down(kmax, o, 64);
+    kmin = a < kmin ? a : kmin;
+    kmax = b > kmax ? b : kmax;
+  }
+  if ((t & 63) == 0) {
+    s_mm[2 * (t >> 6)] = kmin;
+    s_mm[2 * (t >> 6) + 1] = kmax;
+  }
+#pragma unroll
+  for (int i = 0; i < NQ / WT; ++i) s.bins[i * WT + t] = 0;
+  __syncthreads();
+  kmin = ~0ull;
+  kmax = 0ull;
+#pragma unroll
+  for (int i = 0; i < NWAVE; ++i) {
+    kmin = s_mm[2 * i] < kmin ? s_mm[2 * i] : kmin;
+    kmax = s_mm[2 * i + 1] > kmax ? s_mm[2 * i + 1] : kmax;
+  }
+  // Bin of a sheet: monotone non-decreasing in its position.  Uniform in x
+  // when the double range is finite (the usual case), uniform in key space
+  // otherwise.
+  const double xmin = decode_pos(kmin), xmax = decode_pos(kmax);
+  const double range = xmax - xmin;
+  const bool dbl = range > 0.0 && range <= 1.0e300;
+  const double scale = dbl ? (double)NQ / range : 0.0;
+  int kshift = 0;
+  if (!dbl) {
+    const u64 kr = kmax - kmin;
+    while (kshift < 63 && (kr >> kshift) >= (u64)NQ) ++kshift;
+  }
+  u32 q[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const u32 e = i * WT + t;
+    q[i] = 0;
+    if (e < c) {
+      const u64 k = s.key[e];
+      u32 b;
+      if (dbl) {
+        const double qd = (decode_pos(k) - xmin) * scale;
+        b = qd < (double)NQ ? (u32)qd : (u32)(NQ - 1);
+      } else {
+        b = (u32)((k - kmin) >> kshift);
+      }
+      q[i] = b;
+      atomicAdd(&s.bins[b], 1u);
+    }
+  }
+  __syncthreads();
+  if (prof && t == 0) prof[6] = wall_clock64();
+  {
+    constexpr int BPT = NQ / WT;  // bins t*BPT .. t*BPT + BPT - 1
+    u32 loc[BPT];
+    u64 sum = 0;
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      loc[i] = s.bins[t * BPT + i];
+      sum += loc[i];
+    }
+    u64 run = block_excl_u64<NWAVE>(sum, s_red, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      s.bins[t * BPT + i] = (u32)run;
+      run += loc[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const u32 e = i * WT + t;
+    if (e < c) s.ord[atomicAdd(&s.bins[q[i]], 1u)] = (unsigned short)e;
+  }
+  __syncthreads();
+  if (prof && t == 0) prof[7] = wall_clock64();
+  // bins[b] is now the end of bin b; insertion-sort bins holding >= 2 sheets
+  {
+    constexpr int BPT = NQ / WT;
+    u32 lo = t == 0 ? 0u : s.bins[t * BPT - 1];
+    for (int i = 0; i < BPT; ++i) {
+      const u32 hi = s.bins[t * BPT + i];
+      for (u32 a = lo + 1; a < hi; ++a) {
+        const unsigned short ea = s.ord[a];
+        const u64 ka = s.key[ea];
+        const i32 ia = s.id[ea];
+        u32 p = a;
+        while (p > lo) {
+          const unsigned short eb = s.ord[p - 1];
+          if (!pair_lt(ka, ia, s.key[eb], s.id[eb])) break;
+          s.ord[p] = eb;
+          --p;
+        }
+        s.ord[p] = ea;
+      }
+      lo = hi;
+    }
+  }
+  __syncthreads();
+}
+
+
+}  // namespace wdy

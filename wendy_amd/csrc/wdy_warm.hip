@@ -45,6 +45,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "wdy_lds.h"
 #include "wdy_warm.h"
 
 namespace wdy {
@@ -91,194 +92,6 @@ __device__ __forceinline__ void publish(u64 *st, u64 ep, u32 flag, u32 cnt,
 __device__ void set_err(u32 *err, u32 epoch, u32 kind) {
   atomicCAS(err, 0u, epoch);
   atomicOr(err + 1, kind);
-}
-
-__device__ __forceinline__ u64 mass_fix(double m, double fscale) {
-  return (u64)llrint(m * fscale);
-}
-
-// Exclusive scan of one u64 per thread over the block; *total gets the sum.
-template <int NWAVE>
-__device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
-  const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  u64 x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const u64 y = __shfl_up(x, o, 64);
-    if (lane >= (u32)o) x += y;
-  }
-  __syncthreads();
-  if (lane == 63) s_red[w] = x;
-  __syncthreads();
-  u64 wp = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < NWAVE; ++i) {
-    if (i < (int)w) wp += s_red[i];
-    tot += s_red[i];
-  }
-  if (total) *total = tot;
-  return wp + x - v;
-}
-
-// Exclusive scan of s[0..n) (n <= threads) in place; returns the total.
-template <int NWAVE>
-__device__ u32 block_excl_u32_inplace(u32 *s, u32 n, u64 *s_red) {
-  const u32 t = threadIdx.x;
-  const u32 v = t < n ? s[t] : 0u;
-  u64 tot;
-  const u64 e = block_excl_u64<NWAVE>(v, s_red, &tot);
-  __syncthreads();
-  if (t < n) s[t] = (u32)e;
-  __syncthreads();
-  return (u32)tot;
-}
-
-__device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
-  return ka < kb || (ka == kb && ia < ib);
-}
-
-// One tile shape: threads per workgroup and the most sheets a bucket holds.
-template <int WT_, int CAP_>
-struct Shape {
-  static constexpr int WT = WT_, CAP = CAP_, NQ = CAP_, PER = CAP_ / WT_;
-  static constexpr int NWAVE = WT_ / 64;
-  static_assert(CAP_ % WT_ == 0 && WT_ % 64 == 0 && CAP_ <= 65535, "shape");
-};
-
-template <class SH, bool EQ>
-struct Lds {
-  static constexpr int CAP = SH::CAP;
-  static constexpr size_t BYTES =
-      (size_t)CAP * (EQ ? 22 : 30) + (size_t)SH::NQ * 4;
-  u64 *key;
-  double *v;
-  double *m;  // unequal masses only
-  i32 *id;
-  unsigned short *ord;
-  u32 *bins;
-  __device__ explicit Lds(u64 *base) {
-    key = base;
-    v = (double *)(key + CAP);
-    m = v + CAP;
-    id = (i32 *)(EQ ? (v + CAP) : (m + CAP));
-    ord = (unsigned short *)(id + CAP);
-    bins = (u32 *)(ord + CAP);
-  }
-};
-
-// s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
-template <class SH, bool EQ>
-__device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm) {
-  constexpr int WT = SH::WT, PER = SH::PER, NQ = SH::NQ, NWAVE = SH::NWAVE;
-  const u32 t = threadIdx.x;
-  u64 kmin = ~0ull, kmax = 0ull;
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const u32 e = i * WT + t;
-    if (e < c) {
-      const u64 k = s.key[e];
-      kmin = k < kmin ? k : kmin;
-      kmax = k > kmax ? k : kmax;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const u64 a = __shfl_down(kmin, o, 64), b = __shfl_down(kmax, o, 64);
-    kmin = a < kmin ? a : kmin;
-    kmax = b > kmax ? b : kmax;
-  }
-  if ((t & 63) == 0) {
-    s_mm[2 * (t >> 6)] = kmin;
-    s_mm[2 * (t >> 6) + 1] = kmax;
-  }
-#pragma unroll
-  for (int i = 0; i < NQ / WT; ++i) s.bins[i * WT + t] = 0;
-  __syncthreads();
-  kmin = ~0ull;
-  kmax = 0ull;
-#pragma unroll
-  for (int i = 0; i < NWAVE; ++i) {
-    kmin = s_mm[2 * i] < kmin ? s_mm[2 * i] : kmin;
-    kmax = s_mm[2 * i + 1] > kmax ? s_mm[2 * i + 1] : kmax;
-  }
-  // Bin of a sheet: monotone non-decreasing in its position.  Uniform in x
-  // when the double range is finite (the usual case), uniform in key space
-  // otherwise.
-  const double xmin = decode_pos(kmin), xmax = decode_pos(kmax);
-  const double range = xmax - xmin;
-  const bool dbl = range > 0.0 && range <= 1.0e300;
-  const double scale = dbl ? (double)NQ / range : 0.0;
-  int kshift = 0;
-  if (!dbl) {
-    const u64 kr = kmax - kmin;
-    while (kshift < 63 && (kr >> kshift) >= (u64)NQ) ++kshift;
-  }
-  u32 q[PER];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const u32 e = i * WT + t;
-    q[i] = 0;
-    if (e < c) {
-      const u64 k = s.key[e];
-      u32 b;
-      if (dbl) {
-        const double qd = (decode_pos(k) - xmin) * scale;
-        b = qd < (double)NQ ? (u32)qd : (u32)(NQ - 1);
-      } else {
-        b = (u32)((k - kmin) >> kshift);
-      }
-      q[i] = b;
-      atomicAdd(&s.bins[b], 1u);
-    }
-  }
-  __syncthreads();
-  {
-    constexpr int BPT = NQ / WT;  // bins t*BPT .. t*BPT + BPT - 1
-    u32 loc[BPT];
-    u64 sum = 0;
-#pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-      loc[i] = s.bins[t * BPT + i];
-      sum += loc[i];
-    }
-    u64 run = block_excl_u64<NWAVE>(sum, s_red, nullptr);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-      s.bins[t * BPT + i] = (u32)run;
-      run += loc[i];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const u32 e = i * WT + t;
-    if (e < c) s.ord[atomicAdd(&s.bins[q[i]], 1u)] = (unsigned short)e;
-  }
-  __syncthreads();
-  // bins[b] is now the end of bin b; insertion-sort bins holding >= 2 sheets
-  {
-    constexpr int BPT = NQ / WT;
-    u32 lo = t == 0 ? 0u : s.bins[t * BPT - 1];
-    for (int i = 0; i < BPT; ++i) {
-      const u32 hi = s.bins[t * BPT + i];
-      for (u32 a = lo + 1; a < hi; ++a) {
-        const unsigned short ea = s.ord[a];
-        const u64 ka = s.key[ea];
-        const i32 ia = s.id[ea];
-        u32 p = a;
-        while (p > lo) {
-          const unsigned short eb = s.ord[p - 1];
-          if (!pair_lt(ka, ia, s.key[eb], s.id[eb])) break;
-          s.ord[p] = eb;
-          --p;
-        }
-        s.ord[p] = ea;
-      }
-      lo = hi;
-    }
-  }
-  __syncthreads();
 }
 
 // The bucket of key k: the largest b < nb with spl[b] <= k (spl[0] = 0).
@@ -425,7 +238,8 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
     WARM_STAMP(1);
 
     // ---- 4. sort the bucket in LDS, while the predecessors publish ---------
-    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm);
+    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm,
+                                    A.prof ? A.prof + 8ull * j : nullptr);
     else __syncthreads();
     WARM_STAMP(2);
 
@@ -745,7 +559,8 @@ struct WarmCtx {
   // optional per-phase profile (environment variable WDY_PROFILE_WARM=1)
   u64 *prof = nullptr;
   size_t prof_alloc = 0;
-  double prof_us[6] = {};  // gather, sort, look-back, force, write, span
+  double prof_us[9] = {};  // gather, sort, look-back, force, write, span,
+                           // sort: min/max+histogram, scan+scatter, fix-ups
   int64_t prof_n = 0;
   double clk_khz = 100000.0;
 };
@@ -780,10 +595,12 @@ void warm_free(wdy_sim *s) {
       fprintf(stderr,
               "wendy_amd warm-step profile (n=%lld, %lld launches, mean us "
               "per bucket): gather %.2f sort %.2f look-back %.2f force %.2f "
-              "partition+write %.2f | launch span %.1f\n",
+              "partition+write %.2f | launch span %.1f | sort = histogram "
+              "%.2f + scan/scatter %.2f + fix-ups %.2f\n",
               (long long)s->n, (long long)w->prof_n, w->prof_us[0] * k,
               w->prof_us[1] * k, w->prof_us[2] * k, w->prof_us[3] * k,
-              w->prof_us[4] * k, w->prof_us[5] * k);
+              w->prof_us[4] * k, w->prof_us[5] * k, w->prof_us[6] * k,
+              w->prof_us[7] * k, w->prof_us[8] * k);
     }
     (void)hipFree(w->prof);
   }
@@ -858,14 +675,18 @@ static void prof_fold(wdy_sim *s) {
     return;
   const double us = 1000.0 / w.clk_khz;
   u64 t0 = ~0ull, t5 = 0;
-  double d[5] = {};
+  double d[8] = {};
   for (u32 j = 0; j < w.nb; ++j) {
     const u64 *q = &p[8ull * j];
     for (int k = 0; k < 5; ++k) d[k] += (double)(q[k + 1] - q[k]) * us;
+    d[5] += (double)(q[6] - q[1]) * us;
+    d[6] += (double)(q[7] - q[6]) * us;
+    d[7] += (double)(q[2] - q[7]) * us;
     t0 = q[0] < t0 ? q[0] : t0;
     t5 = q[5] > t5 ? q[5] : t5;
   }
   for (int k = 0; k < 5; ++k) w.prof_us[k] += d[k] / w.nb;
+  for (int k = 5; k < 8; ++k) w.prof_us[k + 1] += d[k] / w.nb;
   w.prof_us[5] += (double)(t5 - t0) * us;
   w.prof_n += 1;
 }
