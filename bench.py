@@ -141,6 +141,80 @@ def bench_ensemble(args, log):
             'speedup': rate_b / rate_p, 'unit': 'sheet-steps/s'}
 
 
+def serve_workload(window_s, rate=200., seed=0, pool_log2=21):
+    """Requests of the serving load: Poisson arrivals at ``rate``/s over
+    ``window_s`` seconds; N log-uniform in [2^10, 2^20]; z_h ~ U(0.5, 2),
+    sigma = sqrt(z_h); lognormal(0, 0.3) masses normalised to 1; omega ~
+    U(0, 0.2); duration log-uniform in [100, 10000] leapfrog steps of h = 0.01.
+    Initial conditions are windows of one pre-drawn sample (sech^2 unit
+    positions, normal velocities, lognormal masses; 2^21 sheets) at random
+    offsets, scaled per request -- cheap enough to make at arrival time."""
+    from wendy_amd import serve
+    rs = numpy.random.RandomState(seed)
+    t, arr = 0., []
+    while True:
+        t += rs.exponential(1. / rate)
+        if t > window_s:
+            break
+        arr.append(t)
+    k = len(arr)
+    ns = numpy.floor(2. ** rs.uniform(10, 20, k)).astype(int)
+    zhs = rs.uniform(0.5, 2., k)
+    oms = rs.uniform(0., 0.2, k)
+    steps = numpy.floor(10. ** rs.uniform(2, 4, k)).astype(int)
+    offs = rs.randint(0, 1 << 30, size=(k, 3))
+    base = numpy.random.RandomState(seed + 1)
+    P = 1 << pool_log2
+    u = base.uniform(size=P)
+    u[u == 0.] = 0.5
+    xb = 2. * numpy.arctanh(2. * u - 1.)
+    gb = base.normal(size=P)
+    lb = base.lognormal(0., 0.3, size=P)
+
+    def maker(n, zh, o):
+        def make():
+            a, b, c = (int(q) % (P - n + 1) for q in o)
+            m = lb[c:c + n].copy()
+            return zh * xb[a:a + n], numpy.sqrt(zh) * gb[b:b + n], m / m.sum()
+        return make
+    return [serve.Request(i, int(ns[i]), maker(int(ns[i]), zhs[i], offs[i]),
+                          int(steps[i]), 0.01, omega=float(oms[i]), arrival=arr[i])
+            for i in range(k)]
+
+
+def bench_serve(args, log):
+    """Continuous batching of the serving load on one GPU (wendy_amd.serve):
+    a 200 GB HBM budget, pool of 2048 slots for N <= 2^16, larger requests on
+    the large path (up to 8 at once), ticks of 10 steps."""
+    from wendy_amd import serve
+    reqs = serve_workload(args.serve_window, args.serve_rate)
+    offered = sum(r.n * r.steps for r in reqs) / args.serve_window
+    log('serve: %d requests over %.1f s (offered %.3e sheet-steps/s)'
+        % (len(reqs), args.serve_window, offered))
+    s = serve.Server(budget_bytes=200e9, max_sims=2048, split=1 << 16,
+                     max_big=8, tick=10)
+    try:
+        res = s.run(reqs)
+    finally:
+        s.close()
+    summ = serve.latency_summary(res)
+    log('serve: p50 %.3f s p99 %.3f s, %.3e sheet-steps/s, %s'
+        % (summ['p50_s'], summ['p99_s'], summ['sheet_steps_per_s'], s.stats))
+    where = [r.where for r in res.values()]
+    return {'workload': ('Poisson arrivals %.0f/s for %.1f s, N log-uniform '
+                         '[2^10, 2^20], lognormal(0,0.3) masses, z_h~U(0.5,2), '
+                         'sigma=sqrt(z_h), omega~U(0,0.2), 100-10000 steps '
+                         'log-uniform, h=0.01, 200 GB HBM budget'
+                         % (args.serve_rate, args.serve_window)),
+            'requests': len(reqs), 'offered_sheet_steps_per_s': offered,
+            'p50_latency_s': summ['p50_s'], 'p99_latency_s': summ['p99_s'],
+            'mean_latency_s': summ['mean_s'],
+            'sheet_steps_per_s': summ['sheet_steps_per_s'],
+            'pool_requests': where.count('pool'), 'large_path_requests': where.count('sim'),
+            'ticks': s.stats['ticks'], 'admission_stalls': s.stats['admission_stalls'],
+            'wall_s': s.stats['wall_s']}
+
+
 def cpu_baseline(x, v, m, dt, nleap, omega, threads, budget_s=25.):
     """Time the reference C (sort='parallel') on a bounded sample.
 
@@ -191,6 +265,9 @@ def main():
     ap.add_argument('--ens-sims', type=int, default=4096)
     ap.add_argument('--ens-steps', type=int, default=10)
     ap.add_argument('--ens-sample', type=int, default=256)
+    ap.add_argument('--no-serve', action='store_true')
+    ap.add_argument('--serve-window', type=float, default=3.)
+    ap.add_argument('--serve-rate', type=float, default=200.)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; --gpus %d is not '
@@ -263,6 +340,7 @@ def main():
                 'step_frac_B40': rate * 40 / HBM_PEAK}
 
     ens = None if args.no_ensemble else bench_ensemble(args, log)
+    srv = None if args.no_serve else bench_serve(args, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -287,6 +365,7 @@ def main():
         'roofline': roofline,
         'cpu_baseline': cpu,
         'ensemble': ens,
+        'serving': srv,
     }
     print(json.dumps(out))
 

@@ -175,6 +175,43 @@ int wdy_batch_read(wdy_batch *batch, double *x_out, double *v_out,
                    double *seconds_out);
 int wdy_batch_destroy(wdy_batch *batch);
 
+/* ---- serving pool ----------------------------------------------------------
+ * Continuous batching: simulations join and leave while others run.  The
+ * pool owns an arena of capacity_sheets sheets (56 B of HBM per sheet: two
+ * buffer sets of key, velocity, index and mass) and max_sims slots; every
+ * wdy_pool_step advances a chosen subset of live slots, each by its own h and
+ * step count (one output interval: half drift, nleap x (sort, force, kick,
+ * drift), last drift halved -- as wdy_advance), in one launch.  Uploads by
+ * wdy_pool_add run on a second stream and overlap a running step.  A
+ * simulation's result does not depend on what else shares the pool.  No
+ * reference counterpart.  Calls on one pool must be serialised. */
+typedef struct wdy_pool wdy_pool;
+/* max_n: largest simulation the pool accepts (sizes the staging buffers). */
+int wdy_pool_create(int64_t capacity_sheets, int max_sims, int64_t max_n,
+                    wdy_pool **out);
+/* Admit one simulation (m already x twopiG, totmass = numpy.sum(m), omega2 <
+ * 0 for none).  WDY_ENOMEM when no slot or no free arena range of n sheets is
+ * left: the caller keeps the request queued and retries after a removal.
+ * The host arrays must stay valid until the next wdy_pool_sync or
+ * wdy_pool_read returns. */
+int wdy_pool_add(wdy_pool *pool, int64_t n, const double *x, const double *v,
+                 const double *m, double totmass, double omega2, int *slot_out);
+/* Asynchronous: slot slots[k] advances by one interval of nleap[k] steps of
+ * size h[k] (nleap[k] = 0: untouched). */
+int wdy_pool_step(wdy_pool *pool, int count, const int *slots, const double *h,
+                  const int *nleap);
+/* De-sorted x, v of one slot (either may be NULL); blocks until every step
+ * enqueued before it is done. */
+int wdy_pool_read(wdy_pool *pool, int slot, double *x_out, double *v_out);
+/* Free a slot and its arena range (reused only after the steps enqueued so
+ * far have finished). */
+int wdy_pool_remove(wdy_pool *pool, int slot);
+/* Wait for all uploads and steps; seconds_out: device time of the last step. */
+int wdy_pool_sync(wdy_pool *pool, double *seconds_out);
+int wdy_pool_stats(wdy_pool *pool, int *live, int64_t *used_sheets,
+                   int64_t *largest_free);
+int wdy_pool_destroy(wdy_pool *pool);
+
 /* ---- reference-signature compatibility entry point ----------------------
  * Same name, arguments and meaning as the reference's
  * _wendy_nbody_approx_onestep (wendy/wendy.c:385-393, wendy/wendy.h:29-34),

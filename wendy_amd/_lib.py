@@ -37,6 +37,9 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
             'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
             'wdy_batch_destroy',
+            'wdy_pool_create', 'wdy_pool_add', 'wdy_pool_step', 'wdy_pool_read',
+            'wdy_pool_remove', 'wdy_pool_sync', 'wdy_pool_stats',
+            'wdy_pool_destroy',
             '_wendy_nbody_approx_onestep')
 
 _F64 = ndpointer(dtype=numpy.float64, flags=('C_CONTIGUOUS',))
@@ -104,6 +107,27 @@ def _declare(lib):
     lib.wdy_batch_read.argtypes = [S, S, S, _c_double_p]
     lib.wdy_batch_destroy.restype = ctypes.c_int
     lib.wdy_batch_destroy.argtypes = [S]
+    _I32 = ndpointer(dtype=numpy.int32, flags=('C_CONTIGUOUS',))
+    _c_int_p = ctypes.POINTER(ctypes.c_int)
+    _c_i64_p = ctypes.POINTER(ctypes.c_int64)
+    lib.wdy_pool_create.restype = ctypes.c_int
+    lib.wdy_pool_create.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int64,
+                                    ctypes.POINTER(S)]
+    lib.wdy_pool_add.restype = ctypes.c_int
+    lib.wdy_pool_add.argtypes = [S, ctypes.c_int64, _F64, _F64, _F64,
+                                 ctypes.c_double, ctypes.c_double, _c_int_p]
+    lib.wdy_pool_step.restype = ctypes.c_int
+    lib.wdy_pool_step.argtypes = [S, ctypes.c_int, _I32, _F64, _I32]
+    lib.wdy_pool_read.restype = ctypes.c_int
+    lib.wdy_pool_read.argtypes = [S, ctypes.c_int, S, S]
+    lib.wdy_pool_remove.restype = ctypes.c_int
+    lib.wdy_pool_remove.argtypes = [S, ctypes.c_int]
+    lib.wdy_pool_sync.restype = ctypes.c_int
+    lib.wdy_pool_sync.argtypes = [S, _c_double_p]
+    lib.wdy_pool_stats.restype = ctypes.c_int
+    lib.wdy_pool_stats.argtypes = [S, _c_int_p, _c_i64_p, _c_i64_p]
+    lib.wdy_pool_destroy.restype = ctypes.c_int
+    lib.wdy_pool_destroy.argtypes = [S]
     lib.wdy_potential_arrays.restype = ctypes.c_int
     lib.wdy_potential_arrays.argtypes = [ctypes.c_int64, _F64, ctypes.c_int64,
                                          _F64, _F64, ctypes.c_double,

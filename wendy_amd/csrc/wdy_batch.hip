@@ -30,26 +30,10 @@
 #include <new>
 #include <vector>
 
+#include "wdy_batch.h"
 #include "wdy_lds.h"
 
 namespace wdy {
-
-struct BatchDesc {
-  u64 off;        // first arena slot
-  u32 n;          // sheets
-  u32 flags;      // 1 equal masses, 2 harmonic term
-  int cur;        // current buffer set
-  int nleap;      // leapfrog steps this interval (0: skip)
-  double h;       // leapfrog step
-  double mq, totmass, omega2, fscale, finv;
-};
-
-struct BatchArrays {
-  u64 *key[2];
-  double *vel[2];
-  i32 *idx[2];
-  double *mass[2];
-};
 
 namespace {
 
@@ -266,6 +250,26 @@ __global__ __launch_bounds__(BWT) void k_batch_advance(
 
 }  // namespace
 
+void batch_desc_fill(BatchDesc &d, const double *m, u32 n, double totmass,
+                     double omega2) {
+  bool eq = true;
+  for (u32 i = 1; i < n && eq; ++i) eq = m[i] == m[0];
+  d.off = 0;
+  d.n = n;
+  d.flags = (eq ? 1u : 0u) | (omega2 >= 0.0 ? 2u : 0u);
+  d.cur = 0;
+  d.nleap = 0;
+  d.h = 0.0;
+  d.mq = m[0];
+  d.totmass = totmass;
+  d.omega2 = omega2 >= 0.0 ? omega2 : -1.0;
+  // fixed-point scale of the exact prefix: 2^k * totmass <= 2^62
+  int e = 0;
+  (void)frexp(totmass > 0.0 ? totmass : 1.0, &e);
+  d.fscale = ldexp(1.0, 62 - e);
+  d.finv = ldexp(1.0, e - 62);
+}
+
 int batch_launch(hipStream_t st, const BatchDesc *d_descs, int count,
                  const BatchArrays &arr, u32 *err) {
   if (count <= 0) return WDY_OK;
@@ -396,23 +400,10 @@ int wdy_batch_create(int64_t nsims, const int64_t *offsets, const double *x,
     std::vector<BatchDesc> descs((size_t)nsims);
     for (int64_t k = 0; k < nsims; ++k) {
       BatchDesc &d = descs[(size_t)k];
+      batch_desc_fill(d, m + offsets[k], (u32)(offsets[k + 1] - offsets[k]),
+                      totmass[k], omega2[k]);
       d.off = (u64)offsets[k];
-      d.n = (u32)(offsets[k + 1] - offsets[k]);
-      bool eq = true;
-      for (int64_t i = offsets[k] + 1; i < offsets[k + 1]; ++i)
-        eq = eq && m[i] == m[offsets[k]];
-      any_unequal = any_unequal || !eq;
-      d.flags = (eq ? 1u : 0u) | (omega2[k] >= 0.0 ? 2u : 0u);
-      d.cur = 0;
-      d.nleap = 0;
-      d.h = 0.0;
-      d.mq = m[offsets[k]];
-      d.totmass = totmass[k];
-      d.omega2 = omega2[k] >= 0.0 ? omega2[k] : -1.0;
-      int e = 0;
-      (void)frexp(totmass[k] > 0.0 ? totmass[k] : 1.0, &e);
-      d.fscale = ldexp(1.0, 62 - e);
-      d.finv = ldexp(1.0, e - 62);
+      any_unequal = any_unequal || !(d.flags & 1u);
     }
     std::stable_sort(descs.begin(), descs.end(),
                      [](const BatchDesc &p, const BatchDesc &q) { return p.n > q.n; });
