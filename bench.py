@@ -250,6 +250,21 @@ def cpu_baseline(x, v, m, dt, nleap, omega, threads, budget_s=25.):
                        "reference's own wall clock" % (n, threads, steps, secs))}
 
 
+def copy_probe(mib=1024, launches=5):
+    """A plain device copy of known size (torch elementwise kernel: read
+    mib MiB, write mib MiB per launch) whose HBM counters calibrate the
+    FETCH_SIZE / WRITE_SIZE readings of the kernels above."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    for _ in range(launches):
+        torch.add(a, 1., out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_probe_read_bytes': n * 8, 'copy_probe_write_bytes': n * 8,
+                      'launches': launches}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -266,6 +281,8 @@ def main():
     ap.add_argument('--ens-steps', type=int, default=10)
     ap.add_argument('--ens-sample', type=int, default=256)
     ap.add_argument('--no-serve', action='store_true')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='only run the calibration copy kernel')
     ap.add_argument('--serve-window', type=float, default=3.)
     ap.add_argument('--serve-rate', type=float, default=200.)
     args = ap.parse_args()
@@ -275,6 +292,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe()
+        return
     import wendy_amd
     n, nleap, dt, omega = args.n, args.nleap, args.dt, args.omega
     x, v, m = workload(n)

@@ -155,3 +155,28 @@ def test_warm_unequal_deterministic(gpu_lib):
         assert sim.path_stats()[0] == 10
         sim.close()
     assert bits_equal(res[0][0], res[1][0]) and bits_equal(res[0][1], res[1][1])
+
+
+@pytest.mark.parametrize('shape', [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize('mass', ['equal', 'lognormal'])
+def test_warm_every_shape(gpu_lib, shape, mass):
+    """Every tile shape (threads, bucket capacity, position bins) gives the
+    oracle's trajectory."""
+    print('shape', shape, mass, flush=True)
+    x, v, m = make(40000, 10 + shape, mass)
+    warm, cold, redo = run_both(x, v, m, 0.01, 5, 3, omega=0.1, tune=(0, 8, shape))
+    assert warm == 10 and cold == 5 and redo == 0
+
+
+def test_warm_window_limit_per_shape(gpu_lib):
+    x, v, m = make(20000, 9)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large')
+    try:
+        sim.tune(0, 31, 6)
+        with pytest.raises(ValueError):
+            sim.tune(0, 32, 6)
+        with pytest.raises(ValueError):
+            sim.tune(0, 64, 3)
+        sim.tune(0, 64, 0)
+    finally:
+        sim.close()

@@ -376,10 +376,10 @@ int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {
     return fail(WDY_EUNSUPPORTED, "wdy_tune: the sim has no warm-start path");
   if (shape < -1 || shape >= WARM_NSHAPES)
     return fail(WDY_EINVAL, "wdy_tune: shape %d out of range", shape);
-  const int cap = warm_shape(shape >= 0 ? shape : warm_current_shape(sim)).cap;
+  const WarmShape &sh = warm_shape(shape >= 0 ? shape : warm_current_shape(sim));
   if ((bucket_target != 0 &&
-       (bucket_target < (int)WDY_BUCKET_MIN || bucket_target > cap)) ||
-      window < 0 || window > WDY_WINDOW_MAX)
+       (bucket_target < (int)WDY_BUCKET_MIN || bucket_target > sh.cap)) ||
+      window < 0 || window > sh.wmax())
     return fail(WDY_EINVAL, "wdy_tune: bucket_target %d / window %d out of range",
                 bucket_target, window);
   WDY_HIP(hipSetDevice(sim->device));

@@ -53,11 +53,14 @@ __device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
 }
 
 // One tile shape: threads per workgroup and the most sheets a bucket holds.
-template <int WT_, int CAP_>
+// WT threads, CAP sheets per bucket, NQ position bins for the bucket sort
+// (also the per-wave segment counters of the warm step's partition).
+template <int WT_, int CAP_, int NQ_ = CAP_>
 struct Shape {
-  static constexpr int WT = WT_, CAP = CAP_, NQ = CAP_, PER = CAP_ / WT_;
+  static constexpr int WT = WT_, CAP = CAP_, NQ = NQ_, PER = CAP_ / WT_;
   static constexpr int NWAVE = WT_ / 64;
-  static_assert(CAP_ % WT_ == 0 && WT_ % 64 == 0 && CAP_ <= 65535, "shape");
+  static_assert(CAP_ % WT_ == 0 && NQ_ % WT_ == 0 && WT_ % 64 == 0 &&
+                    CAP_ <= 65535, "shape");
 };
 
 template <class SH, bool EQ>
@@ -174,27 +177,31 @@ __device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
   }
   __syncthreads();
   if (prof && t == 0) prof[7] = wall_clock64();
-  // bins[b] is now the end of bin b; insertion-sort bins holding >= 2 sheets
-  {
-    constexpr int BPT = NQ / WT;
-    u32 lo = t == 0 ? 0u : s.bins[t * BPT - 1];
-    for (int i = 0; i < BPT; ++i) {
-      const u32 hi = s.bins[t * BPT + i];
-      for (u32 a = lo + 1; a < hi; ++a) {
-        const unsigned short ea = s.ord[a];
-        const u64 ka = s.key[ea];
-        const i32 ia = s.id[ea];
-        u32 p = a;
-        while (p > lo) {
-          const unsigned short eb = s.ord[p - 1];
-          if (!pair_lt(ka, ia, s.key[eb], s.id[eb])) break;
-          s.ord[p] = eb;
-          --p;
-        }
-        s.ord[p] = ea;
+  // bins[b] is now the end of bin b.  Rank each sheet inside its bin by
+  // counting the bin's smaller (key, id) pairs -- every sheet in parallel, a
+  // bin of k sheets costs k comparisons per sheet -- then place it.
+  u32 rk[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const u32 e = i * WT + t;
+    rk[i] = 0;
+    if (e < c) {
+      const u32 lo = q[i] == 0 ? 0u : s.bins[q[i] - 1], hi = s.bins[q[i]];
+      const u64 k = s.key[e];
+      const i32 id = s.id[e];
+      u32 r = lo;
+      for (u32 a = lo; a < hi; ++a) {
+        const unsigned short f = s.ord[a];
+        r += pair_lt(s.key[f], s.id[f], k, id) ? 1u : 0u;
       }
-      lo = hi;
+      rk[i] = r;
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const u32 e = i * WT + t;
+    if (e < c) s.ord[rk[i]] = (unsigned short)e;
   }
   __syncthreads();
 }

@@ -10,9 +10,15 @@ constexpr int WARM_WMAX = 64;          // largest window (buckets either side)
 // most sheets one bucket may hold (its LDS capacity).  The default bucket
 // target is 3/4 of the capacity.
 struct WarmShape {
-  int threads, cap;
+  int threads, cap, nq;
+  // widest window the partition's per-wave segment counters (nq u32 shared
+  // with the sort's bins) can hold: waves * (2W + 2) <= nq
+  int wmax() const {
+    const int w = (nq / (threads / 64) - 2) / 2;
+    return w < WARM_WMAX ? w : WARM_WMAX;
+  }
 };
-constexpr int WARM_NSHAPES = 4;
+constexpr int WARM_NSHAPES = 7;
 constexpr int WARM_SHAPE_DEFAULT = 0;
 const WarmShape &warm_shape(int v);
 constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry

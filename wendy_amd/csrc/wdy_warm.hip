@@ -17,7 +17,7 @@
 //      (wendy/wendy.c:318-323,398), and publishes its sheet count and
 //      fixed-point mass total;
 //   4. sorts the bucket in LDS by (position, sheet index): one counting pass
-//      over position bins, then insertion fix-ups of the (tiny) bins;
+//      over position bins, then every sheet ranks itself inside its bin;
 //   5. resolves its global rank offset and mass prefix with a decoupled
 //      look-back (one wave probing 64 predecessors at a time) over integer
 //      values, so nothing depends on timing;
@@ -513,9 +513,12 @@ int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
 
 }  // namespace
 
-// Tile shapes (threads per workgroup, bucket capacity in sheets).
+// Tile shapes (threads per workgroup, bucket capacity in sheets, position
+// bins).  Shapes 4-6 keep the LDS footprint small enough for two or three
+// workgroups per CU, so one workgroup's memory phases overlap another's sort.
 static const WarmShape kShapes[WARM_NSHAPES] = {
-    {1024, 4096}, {512, 2048}, {256, 1024}, {512, 1024}};
+    {1024, 4096, 4096}, {512, 2048, 2048}, {256, 1024, 1024}, {512, 1024, 1024},
+    {512, 2048, 1024}, {512, 3072, 1024}, {768, 3072, 768}};
 
 const WarmShape &warm_shape(int v) { return kShapes[v]; }
 
@@ -525,7 +528,10 @@ int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
     case 0: return launch_shape<Shape<1024, 4096>>(st, a, equal_mass, harm, mode);
     case 1: return launch_shape<Shape<512, 2048>>(st, a, equal_mass, harm, mode);
     case 2: return launch_shape<Shape<256, 1024>>(st, a, equal_mass, harm, mode);
-    default: return launch_shape<Shape<512, 1024>>(st, a, equal_mass, harm, mode);
+    case 3: return launch_shape<Shape<512, 1024>>(st, a, equal_mass, harm, mode);
+    case 4: return launch_shape<Shape<512, 2048, 1024>>(st, a, equal_mass, harm, mode);
+    case 5: return launch_shape<Shape<512, 3072, 1024>>(st, a, equal_mass, harm, mode);
+    default: return launch_shape<Shape<768, 3072, 768>>(st, a, equal_mass, harm, mode);
   }
 }
 
