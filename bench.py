@@ -250,6 +250,21 @@ def cpu_baseline(x, v, m, dt, nleap, omega, threads, budget_s=25.):
                        "reference's own wall clock" % (n, threads, steps, secs))}
 
 
+def measured_traffic(name):
+    """HBM bytes per launch recorded in a committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size); None
+    when the summary is absent."""
+    path = os.path.join(ROOT, 'profiles', name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_launch:'):
+                    return int(line.split(':')[1]), 'profiles/' + name
+    except OSError:
+        pass
+    return None, None
+
+
 def copy_probe(mib=1024, launches=5):
     """A plain device copy of known size (torch elementwise kernel: read
     mib MiB, write mib MiB per launch) whose HBM counters calibrate the
@@ -351,8 +366,12 @@ def main():
         note = 'approximate'
     avg_s = ms * 1e-3 / max(cnt, 1)
     achieved = bpl / avg_s if avg_s > 0 else 0.
+    traffic, traffic_src = None, None
+    if dom == KC_WARM and n == 1 << 24 and nleap == 20:
+        traffic, traffic_src = measured_traffic('warm_step_n16777216_hbm_traffic.md')
     roofline = {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9,
-                'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': None,
+                'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': traffic,
+                'traffic_source': traffic_src,
                 'kernel': name, 'bytes_per_launch': bpl,
                 'avg_launch_us': avg_s * 1e6, 'launches': cnt,
                 'bytes_note': note,
