@@ -118,6 +118,27 @@ int wdy_destroy(wdy_sim *sim);
 #define WDY_WINDOW_MAX 64
 int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window);
 
+/* External force evaluated on the device, the counterpart of nbody's
+ * ext_force callback (wendy/wendy.py:389-420; the C side adds it at
+ * wendy/wendy.c:362-370 and advances *t0 by h after every force evaluation,
+ * :401-402,407-408).  A callback into host code would break device
+ * residency, so the force is one of a menu of forms, evaluated at the sorted
+ * position x and the step's force time t (t0 at the first force of an
+ * output interval, then +h per step, the reference's clock exactly):
+ *   WDY_EXT_HARMONIC    -p0 * x                              (p0 = omega**2)
+ *   WDY_EXT_ISOTHERMAL  (p0 * tanh((0.5 * x) / p1)) / p1     (amp, zh)
+ *   WDY_EXT_PULSE       p0 * exp(-0.5 * ((t - p1) / p2)**2)  (amp, tc, tau)
+ * params holds p0, p1, p2 (ignored for WDY_EXT_NONE); t0 sets the clock.
+ * The acceleration becomes a = E(x, t) + F; without a force t0 is frozen,
+ * as in the reference.  Applies from the next wdy_advance. */
+#define WDY_EXT_NONE 0
+#define WDY_EXT_HARMONIC 1
+#define WDY_EXT_ISOTHERMAL 2
+#define WDY_EXT_PULSE 3
+int wdy_set_ext_force(wdy_sim *sim, int kind, const double *params, double t0);
+/* The simulation clock t0 (advanced only while an external force is set). */
+int wdy_time(wdy_sim *sim, double *t_out);
+
 /* Leapfrog steps taken so far by the warm-start kernel and by the radix
  * path, and the number of output intervals redone on the radix path after a
  * warm-start failure.  No reference counterpart. */

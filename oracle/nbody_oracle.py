@@ -44,17 +44,37 @@ def exclusive_mass_prefix(m_sorted):
     return out
 
 
-def force_sorted(val, m_sorted, totmass, omega2):
+def force_sorted(val, m_sorted, totmass, omega2, ext=None):
     """Acceleration of each sheet in sorted order (wendy/wendy.c:371-383).
 
     ``m_sorted`` is already multiplied by twopiG; ``omega2 < 0`` means no
-    harmonic term (the -1 sentinel of wendy/wendy.py:363-366).
+    harmonic term (the -1 sentinel of wendy/wendy.py:363-366).  ``ext``: the
+    external force in sorted order, which the reference stores in a[] before
+    adding the self-gravity term (wendy/wendy.c:362-370); else a[] is zeroed.
     """
     cumul = exclusive_mass_prefix(m_sorted)
     f = totmass - 2. * cumul - m_sorted
     if omega2 >= 0:
         f = f - omega2 * val
-    return 0.0 + f
+    return (0.0 if ext is None else ext) + f
+
+
+EXTERNAL_SWITCH = 10  # wendy/wendy.h:9-11
+
+
+def ext_force_sorted(ext_force, val, idx, x, t):
+    """The external force at the sorted positions, evaluated the way
+    wendy/wendy.c:362-370 does with the ctypes wrapper of wendy/wendy.py:
+    406-418: for N > EXTERNAL_SWITCH the positions are de-sorted into x and
+    the callable is evaluated once on the array; otherwise once per sheet on
+    the scalar position, in sorted order."""
+    n = len(val)
+    if n > EXTERNAL_SWITCH:
+        x[idx] = val
+        a = numpy.empty(n)
+        a[:] = ext_force(x, t)
+        return a[idx]
+    return numpy.array([ext_force(float(val[i]), t) for i in range(n)])
 
 
 class ApproxState:
@@ -66,8 +86,11 @@ class ApproxState:
     Setup restates wendy/wendy.py:363-387 and :422.
     """
 
-    def __init__(self, x, v, m, dt, nleap, omega=None, twopiG=1.):
+    def __init__(self, x, v, m, dt, nleap, omega=None, twopiG=1.,
+                 ext_force=None, t0=0.):
         self.omega2 = -1. if omega is None else omega ** 2.
+        self.ext_force = ext_force
+        self.t = float(t0)  # the *t0 clock (wendy/wendy.c:401-402,407-408)
         n = len(x)
         self.x = numpy.require(copy.copy(x), dtype=numpy.float64,
                                requirements=['C', 'W'])
@@ -87,8 +110,13 @@ class ApproxState:
         order = canonical_order(self.val, self.idx)
         self.val = self.val[order]
         self.idx = self.idx[order]
+        ext = None
+        if self.ext_force is not None:
+            ext = ext_force_sorted(self.ext_force, self.val, self.idx, self.x,
+                                   self.t)
+            self.t = self.t + dt_kick
         a_sorted = force_sorted(self.val, self.m[self.idx], self.totmass,
-                                self.omega2)
+                                self.omega2, ext)
         # leapfrog_leappq (wendy/wendy.c:324-333)
         vs = self.v[self.idx] + dt_kick * a_sorted
         self.v[self.idx] = vs

@@ -4,8 +4,9 @@ TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
 
 Loads ``oracle/_ref/libwendy_ref*.so`` (built by ``oracle/Makefile`` from the
 reference's own C sources) and drives ``_wendy_nbody_approx_onestep``
-(wendy/wendy.c:385-418, prototype wendy/wendy.h:29-34) with ext_force = NULL,
-preparing its arguments the way wendy/wendy.py:363-433 does.  This does not
+(wendy/wendy.c:385-418, prototype wendy/wendy.h:29-34),
+preparing its arguments the way wendy/wendy.py:363-433 does; an external
+force is passed through the plain-ctypes wrapper of wendy/wendy.py:406-418.  This does not
 import the reference's Python package (which needs numba); it restates the
 argument preparation in a few lines.
 """
@@ -64,8 +65,23 @@ class RefApprox:
     """
 
     def __init__(self, x, v, m, dt, nleap, omega=None, twopiG=1.,
-                 sort='quick', variant=''):
+                 sort='quick', variant='', ext_force=None, t0=0.):
         self.lib = load(variant)
+        self.ext_c = None
+        if ext_force is not None:
+            proto = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double),
+                                     ctypes.c_double,
+                                     ctypes.POINTER(ctypes.c_double))
+
+            def arrays(nn, xp, t, ap):
+                if nn == 1:
+                    return ext_force(xp.contents.value, t)
+                xa = numpy.ctypeslib.as_array(xp, shape=(nn,))
+                aa = numpy.ctypeslib.as_array(ap, shape=(nn,))
+                aa[:] = ext_force(xa, t)
+                return 0.
+            self.ext_c = proto(arrays)
         self.omega2 = -1. if omega is None else omega ** 2.
         n = len(x)
         self.n = n
@@ -81,7 +97,7 @@ class RefApprox:
         self.xi = numpy.zeros(n, dtype=XI_DTYPE)
         self.xi['idx'] = numpy.arange(n, dtype=numpy.int32)
         self.xi['val'] = self.x
-        self.t0 = ctypes.c_double(0.)
+        self.t0 = ctypes.c_double(t0)
         self.err = ctypes.c_int(0)
         self.time_elapsed = ctypes.c_double(0.)
         self.h = dt / nleap
@@ -91,7 +107,8 @@ class RefApprox:
     def onestep(self):
         self.lib._wendy_nbody_approx_onestep(
             self.n, self.xi, self.x, self.v, self.m, self.a, self.totmass,
-            self.h, self.nleap, ctypes.byref(self.t0), self.omega2, None,
+            self.h, self.nleap, ctypes.byref(self.t0), self.omega2,
+            None if self.ext_c is None else ctypes.cast(self.ext_c, ctypes.c_void_p),
             self.sort_type, ctypes.byref(self.err),
             ctypes.byref(self.time_elapsed), self.cumulmass)
         return self.x, self.v, self.time_elapsed.value

@@ -169,3 +169,75 @@ def test_planted_mistakes_are_caught():
     }
     for name, f in mistakes.items():
         assert numpy.max(numpy.abs(f - bf)) > 1e-3, name
+
+
+# ---- external force (wendy/wendy.c:362-370, clock at :401-408) ----------------
+def _ext_forces():
+    from wendy_amd import forces
+    return [forces.Harmonic(1.1), forces.Isothermal(-0.7, 0.5),
+            forces.Pulse(0.3, 0.12, 0.05)]
+
+
+def _three():
+    # tests/test_approx_external.py:7-10,27-30 of the reference
+    return (numpy.array([-1.1, 0.1, 1.3]), numpy.array([3., 2., -5.]),
+            numpy.array([1., 2., 3.]))
+
+
+@pytest.mark.skipif(not R.available(), reason='oracle/_ref not built here')
+@pytest.mark.parametrize('k', [0, 1, 2])
+@pytest.mark.parametrize('n', [3, 101])
+def test_oracle_ext_force_matches_reference_library(k, n):
+    """The oracle's external-force step and clock equal the compiled
+    reference driven through the plain ctypes callback, bit for bit, on both
+    sides of EXTERNAL_SWITCH (per-sheet scalar calls vs one array call)."""
+    f = _ext_forces()[k]
+    if n == 3:
+        x, v, m = _three()
+    else:
+        x, v, m = O.sech2_disk(n, seed=4, mass=('lognormal', 0.1))
+    ref = R.RefApprox(x, v, m, 0.05, 40, ext_force=f, t0=0.01)
+    ora = O.ApproxState(x, v, m, 0.05, 40, ext_force=f, t0=0.01)
+    for _ in range(5):
+        rx, rv, _ = ref.onestep()
+        ox, ov = ora.onestep()
+        assert bits_equal(rx, ox) and bits_equal(rv, ov)
+    assert ref.t0.value == ora.t
+
+
+def test_ext_harmonic_equals_omega_for_tracers():
+    """Massless sheets in an external harmonic force move exactly as with the
+    built-in omega term: -omega**2 x + 0 == 0.0 + (0 - omega2 x)."""
+    from wendy_amd import forces
+    x, v, _ = O.sech2_disk(50, seed=5)
+    m = numpy.zeros(50)
+    a = O.ApproxState(x, v, m, 0.1, 7, ext_force=forces.Harmonic(0.8))
+    b = O.ApproxState(x, v, m, 0.1, 7, omega=0.8)
+    for _ in range(4):
+        ax, av = a.onestep()
+        bx, bv = b.onestep()
+        assert bits_equal(ax, bx) and bits_equal(av, bv)
+
+
+def test_ext_pulse_momentum_follows_the_clock():
+    """A uniform acceleration g(t) changes the total momentum by
+    M h sum_k g(t_k) with t_k = t0 + k h (the force time of step k): the
+    self-gravity pairs cancel.  A clock off by half a step fails this."""
+    from wendy_amd import forces
+    f = forces.Pulse(0.5, 0.07, 0.03)
+    x, v, m = O.sech2_disk(200, seed=6)
+    h, nleap, t0 = 0.01, 5, 0.02
+    st = O.ApproxState(x, v, m, h * nleap, nleap, ext_force=f, t0=t0)
+    p0 = math.fsum(m * v)
+    ts = []
+    t = t0
+    for _ in range(3 * nleap):
+        ts.append(t)
+        t = t + h
+    for _ in range(3):
+        st.onestep()
+    dp = math.fsum(m * st.v) - p0
+    want = math.fsum(m) * h * math.fsum(f(0., tk) for tk in ts)
+    assert abs(dp - want) <= 1e-12
+    shifted = math.fsum(m) * h * math.fsum(f(0., tk + h / 2.) for tk in ts)
+    assert abs(dp - shifted) > 1e-6

@@ -5,6 +5,7 @@ re-exports wendy/wendy.py): ``nbody``, ``energy``, ``momentum``, ``potential``.
 The approx=True leapfrog runs on the GPU through libwendy_amd.so (C ABI,
 include/wendy_amd.h); there is no CPU fallback.
 """
+from . import forces  # noqa: F401
 from .batch import Batch  # noqa: F401
 from .nbody import Sim, energy, momentum, nbody, potential  # noqa: F401
 

@@ -35,6 +35,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
+            'wdy_set_ext_force', 'wdy_time',
             'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
             'wdy_batch_destroy',
             'wdy_pool_create', 'wdy_pool_add', 'wdy_pool_step', 'wdy_pool_read',
@@ -107,6 +108,10 @@ def _declare(lib):
     lib.wdy_batch_read.argtypes = [S, S, S, _c_double_p]
     lib.wdy_batch_destroy.restype = ctypes.c_int
     lib.wdy_batch_destroy.argtypes = [S]
+    lib.wdy_set_ext_force.restype = ctypes.c_int
+    lib.wdy_set_ext_force.argtypes = [S, ctypes.c_int, _F64, ctypes.c_double]
+    lib.wdy_time.restype = ctypes.c_int
+    lib.wdy_time.argtypes = [S, _c_double_p]
     _I32 = ndpointer(dtype=numpy.int32, flags=('C_CONTIGUOUS',))
     _c_int_p = ctypes.POINTER(ctypes.c_int)
     _c_i64_p = ctypes.POINTER(ctypes.c_int64)

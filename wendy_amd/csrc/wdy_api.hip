@@ -257,6 +257,8 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
   if (s->small) {
     int rc = launch_small_advance(s, h, nleap, acc_last, cum_last);
     if (rc) return rc;
+    if (s->ext.kind)  // the kernel's clock, replayed (wendy/wendy.c:401-408)
+      for (int ii = 0; ii < nleap; ++ii) s->t0 = s->t0 + h;
   } else {
     // warm interval when the state is partitioned for this h and nothing
     // asks for the last force; on a device-side failure it rewinds and the
@@ -276,12 +278,15 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
     if (rc) return rc;
     u32 T = 0, nb = 0;
     u64 *spl = s->warm ? warm_cold_splitters(s, &T, &nb) : nullptr;
+    double t = s->t0;  // force time of each step (wendy/wendy.c:401-408)
     for (int ii = 0; ii < nleap; ++ii) {
       const bool last = ii == nleap - 1;
       rc = launch_leap(s, h, last ? h / 2. : h, last ? acc_last : nullptr,
-                       last ? cum_last : nullptr, last ? spl : nullptr, T, nb);
+                       last ? cum_last : nullptr, last ? spl : nullptr, T, nb, t);
       if (rc) return rc;
+      t = t + h;
     }
+    if (s->ext.kind) s->t0 = t;
     s->cold_steps += nleap;
     if (s->warm) {
       rc = warm_partition_after_cold(s, h / 2.);
@@ -385,6 +390,40 @@ int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {
   WDY_HIP(hipSetDevice(sim->device));
   WDY_HIP(hipStreamSynchronize(sim->stream));
   return warm_configure(sim, shape, (u32)bucket_target, (u32)window);
+}
+
+int wdy_set_ext_force(wdy_sim *sim, int kind, const double *params, double t0) {
+  if (int rc = check_sim(sim, "wdy_set_ext_force")) return rc;
+  if (kind < WDY_EXT_NONE || kind > WDY_EXT_PULSE)
+    return fail(WDY_EINVAL, "wdy_set_ext_force: unknown kind %d", kind);
+  if (kind != WDY_EXT_NONE && !params)
+    return fail(WDY_EINVAL, "wdy_set_ext_force: params is NULL");
+  if (!std::isfinite(t0)) return fail(WDY_EINVAL, "wdy_set_ext_force: t0 not finite");
+  ExtForce f;
+  f.kind = kind;
+  if (kind != WDY_EXT_NONE) {
+    for (int i = 0; i < 3; ++i)
+      if (!std::isfinite(params[i]))
+        return fail(WDY_EINVAL, "wdy_set_ext_force: params[%d] not finite", i);
+    f.p0 = params[0];
+    f.p1 = params[1];
+    f.p2 = params[2];
+    if ((kind == WDY_EXT_ISOTHERMAL && f.p1 == 0.0) ||
+        (kind == WDY_EXT_PULSE && f.p2 == 0.0))
+      return fail(WDY_EINVAL, "wdy_set_ext_force: zero scale length / width");
+  }
+  WDY_HIP(hipSetDevice(sim->device));
+  WDY_HIP(hipStreamSynchronize(sim->stream));
+  sim->ext = f;
+  sim->t0 = t0;
+  return WDY_OK;
+}
+
+int wdy_time(wdy_sim *sim, double *t_out) {
+  if (int rc = check_sim(sim, "wdy_time")) return rc;
+  if (!t_out) return fail(WDY_EINVAL, "wdy_time: t_out is NULL");
+  *t_out = sim->t0;
+  return WDY_OK;
 }
 
 int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,

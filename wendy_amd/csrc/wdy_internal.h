@@ -49,6 +49,30 @@ __host__ __device__ inline double decode_pos(u64 k) {
   return x;
 }
 
+// External force evaluated on the device: the counterpart of the reference's
+// ext_force callback (wendy/wendy.py:389-420, called at wendy/wendy.c:362-370)
+// for a fixed menu of forms.  a = E(x, t) + F, as the reference stores the
+// callback's value in a[] and then adds the self-gravity term.
+struct ExtForce {
+  int kind = 0;  // WDY_EXT_* of include/wendy_amd.h; 0: none
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+};
+
+__device__ inline double ext_accel(const ExtForce &f, double x, double t) {
+  switch (f.kind) {
+    case 1:  // harmonic: -omega**2 * x, p0 = omega**2
+      return -f.p0 * x;
+    case 2:  // isothermal slab: amp * tanh(0.5 * x / zh) / zh, (amp, zh)
+      return (f.p0 * tanh((0.5 * x) / f.p1)) / f.p1;
+    case 3: {  // uniform pulse: amp * exp(-0.5 * ((t - tc) / tau)**2)
+      const double u = (t - f.p1) / f.p2;
+      return f.p0 * exp(-0.5 * (u * u));
+    }
+    default:
+      return 0.0;
+  }
+}
+
 // Error reporting: every C-ABI entry point returns one of the WDY_E* codes and
 // leaves a message for wdy_last_error() (thread-local).
 void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
@@ -150,6 +174,11 @@ struct wdy_sim {
   wdy::RadixWork radix;
   wdy::WarmCtx *warm = nullptr;  // null: the large path uses radix steps only
 
+  // external force and its clock: t0 advances by h after every force
+  // evaluation while an external force is set (wendy/wendy.c:401-402,407-408)
+  wdy::ExtForce ext;
+  double t0 = 0.0;
+
   int64_t warm_steps = 0, cold_steps = 0, redos = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   wdy::Timer *timer = nullptr;
@@ -163,7 +192,7 @@ int launch_sort_state(wdy_sim *s);   // sort state -> perm/keys, no payload move
 // radix-path leapfrog step: sort, force, kick, drift into the next buffer
 // set; spl_out (may be null) receives the sorted keys at ranks k*T
 int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
-                double *cum_out, u64 *spl_out, u32 T, u32 nb);
+                double *cum_out, u64 *spl_out, u32 T, u32 nb, double t);
 // de-sort the current state: x_out[j], v_out[j] (v_out may be null, and is
 // ignored in small mode, whose velocities are already in sheet order),
 // acc_out[j] = acc[slot of j] when acc is given, m_out[j] (large unequal-mass

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(1024) void k_small_advance(
     double *__restrict__ vel, const double *__restrict__ mass,
     double *__restrict__ cum, double mq, double totmass, double omega2,
     double h, int nleap, double *__restrict__ acc_out,
-    double *__restrict__ cum_out) {
+    double *__restrict__ cum_out, ExtForce ext, double t0) {
   extern __shared__ u64 smem[];
   __shared__ double s_part[1024];
   u64 *ks = smem;
@@ -97,6 +97,7 @@ __global__ __launch_bounds__(1024) void k_small_advance(
     ks[i] = encode_pos(decode_pos(key_g[i]) + hh * vel[id]);
   }
   __syncthreads();
+  double tf = t0;  // time of this step's force (wendy/wendy.c:401-408)
   for (int step = 0; step < nleap; ++step) {
     const bool last = step == nleap - 1;
     const double hd = last ? hh : h;
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(1024) void k_small_advance(
       const double m = EQ ? mq : mass[id];
       double F = (totmass - 2.0 * C) - m;
       if (HARM) F = F - omega2 * x;
-      const double a = 0.0 + F;
+      const double a = (ext.kind ? ext_accel(ext, x, tf) : 0.0) + F;
       const double v = vel[id] + h * a;
       vel[id] = v;
       x = x + hd * v;
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(1024) void k_small_advance(
       if (last && acc_out) acc_out[id] = a;
       if (last && cum_out) cum_out[i] = C;
     }
+    tf = tf + h;
     __syncthreads();
   }
   for (u32 i = t; i < n; i += T) {
@@ -169,7 +171,7 @@ int launch_small_advance(wdy_sim *s, double h, int nleap, double *acc_out,
     fn<<<1, threads, lds, s->stream>>>(n, s->key[0], s->idx[0], s->vel[0],    \
                                        s->mass[0], s->cum, s->m_equal,        \
                                        s->totmass, s->omega2, h, nleap,       \
-                                       acc_out, cum_out);                     \
+                                       acc_out, cum_out, s->ext, s->t0);      \
   } while (0)
   if (s->equal_mass) {
     if (harm) WDY_SMALL(true, true); else WDY_SMALL(true, false);

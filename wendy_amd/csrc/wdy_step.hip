@@ -7,6 +7,7 @@
 //   drift   x = x + h_d * v                       (wendy/wendy.c:322,331)
 //   force   F = ((totmass - 2*C) - m) [- omega2*x] (wendy/wendy.c:378,382-383)
 //           a = 0.0 + F                           (a[] zeroed, then +=)
+//           a = E(x, t) + F with an external force (a[] = E, then +=)
 //   kick    v = v + h_k * a                       (wendy/wendy.c:330)
 // where C is the exclusive prefix of masses in sorted order
 // (wendy/wendy.c:359-360).  Equal-mass runs use C = rank * m, the exact
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(BLOCK) void k_leap(
     i32 *__restrict__ idx_new, double *__restrict__ mass_new,
     double *__restrict__ acc_out, double *__restrict__ cum_out,
     u64 *__restrict__ spl_out, u32 T, u32 nb, double mq, double totmass,
-    double omega2, double hk, double hd) {
+    double omega2, double hk, double hd, ExtForce ext, double t) {
   const u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const u64 k = ks[i];
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_leap(
     C = tied_prefix<EQ>(ks, perm, idx_old, mass_old, cum, n, i, k, id, mq);
   double F = (totmass - 2.0 * C) - m;
   if (HARM) F = F - omega2 * x;
-  const double a = 0.0 + F;
+  const double a = (ext.kind ? ext_accel(ext, x, t) : 0.0) + F;
   v = v + hk * a;
   x = x + hd * v;
   knew[i] = encode_pos(x);
@@ -410,7 +411,7 @@ int launch_sort_state(wdy_sim *s) {
 }
 
 int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
-                double *cum_out, u64 *spl_out, u32 T, u32 nb) {
+                double *cum_out, u64 *spl_out, u32 T, u32 nb, double t) {
   const u32 n = (u32)s->n;
   int rc = launch_sort_state(s);
   if (rc) return rc;
@@ -425,7 +426,7 @@ int launch_leap(wdy_sim *s, double hk, double hd, double *acc_out,
   k_leap<EQ, HARM><<<grid_for(n), BLOCK, 0, s->stream>>>(                       \
       n, s->key[c], s->perm[0], s->vel[c], s->idx[c], s->mass[c], s->cum,       \
       s->key[o], s->vel[o], s->idx[o], s->mass[o], acc_out, cum_out, spl_out,   \
-      T ? T : 1u, nb, s->m_equal, s->totmass, s->omega2, hk, hd)
+      T ? T : 1u, nb, s->m_equal, s->totmass, s->omega2, hk, hd, s->ext, t)
   if (s->equal_mass) {
     if (harm) WDY_LEAP(true, true); else WDY_LEAP(true, false);
   } else {

@@ -60,6 +60,8 @@ struct WarmArgs {
   u32 *err;     // [2] epoch of the first failing launch, failure kinds
   u32 epoch;
   double mq, totmass, omega2, h, pre, hd, hp, fscale, finv;
+  ExtForce ext;  // external force (kind 0: none) evaluated at time t
+  double t;
   u64 *prof;    // null, or [nb * 8] phase timestamps (wall clock) per bucket
 };
 

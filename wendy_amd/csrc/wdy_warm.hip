@@ -347,7 +347,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
             run += mass_fix(me, A.fscale);
           }
           if (HARM) F = F - A.omega2 * x;
-          const double a = 0.0 + F;
+          const double a = (A.ext.kind ? ext_accel(A.ext, x, A.t) : 0.0) + F;
           const double v = s.v[e] + A.h * a;
           x = x + A.hd * v;
           s.v[e] = v;
@@ -720,6 +720,8 @@ static WarmArgs base_args(wdy_sim *s) {
   a.fscale = w.fscale;
   a.finv = w.finv;
   a.h = a.pre = a.hd = a.hp = 0.0;
+  a.ext = s->ext;
+  a.t = s->t0;
   a.prof = nullptr;
   return a;
 }
@@ -796,6 +798,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
   if (int rc = warm_reset_launch(s->stream, w.ring, w.epoch & 3u, w.err))
     return rc;
   int in = start, rdm = mstart;
+  double t = s->t0;  // force time of step k (wendy/wendy.c:401-408)
   for (int k = 0; k < nleap; ++k) {
     const int out = so[k & 1], wrm = mo[k & 1];
     WarmArgs a = base_args(s);
@@ -807,6 +810,8 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     a.pre = k == 0 ? h / 2. : 0.0;
     a.hd = k == nleap - 1 ? h / 2. : h;
     a.hp = k == nleap - 1 ? h / 2. : 0.0;
+    a.t = t;
+    t = t + h;
     a.prof = w.prof;
     if (s->timing) timer_begin(s->timer, s->stream, KC_WARM);
     int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0, 0,
@@ -842,6 +847,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
   }
   s->cur = in;
   w.rd = rdm;
+  if (s->ext.kind) s->t0 = t;
   return WDY_OK;
 }
 

@@ -13,6 +13,7 @@ import ctypes
 import numpy
 
 from . import _lib
+from .forces import ExtForce
 
 # wendy/wendy.py:102 -- every name maps to the same device sort
 _sort_type_dict = {'quick': 0, 'merge': 1, 'tim': 2, 'qsort': 3, 'parallel': 4}
@@ -105,6 +106,20 @@ class Sim:
                                               ctypes.byref(cnt)))
         return ms.value, cnt.value
 
+    def set_ext_force(self, force, t0=0.):
+        """Evaluate ``force`` (a wendy_amd.forces.ExtForce, or None) at every
+        step from the next ``advance``, with the clock set to ``t0``."""
+        kind = 0 if force is None else force.kind
+        p = numpy.array(force.params() if force is not None else (0., 0., 0.),
+                        dtype=numpy.float64)
+        _lib.check(_lib.lib().wdy_set_ext_force(self._h, kind, p, float(t0)))
+
+    def time(self):
+        """The clock t0 (advances only while an external force is set)."""
+        t = ctypes.c_double(0.)
+        _lib.check(_lib.lib().wdy_time(self._h, ctypes.byref(t)))
+        return t.value
+
     def close(self):
         if self._h is not None and self._h.value:
             _lib.lib().wdy_destroy(self._h)
@@ -134,10 +149,15 @@ def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
     reference; unlike the reference, editing them does not feed back into the
     simulation (its state lives on the GPU).
 
-    ``approx=False`` (the exact event-driven solver) and ``ext_force`` are
-    not provided by this library: the first ``next()`` raises
-    NotImplementedError.  ``t0``, ``maxcoll`` and ``warn_maxcoll`` only matter
-    to those and are accepted for signature compatibility.
+    ``ext_force`` must be one of the device-evaluated forms of
+    ``wendy_amd.forces`` (Harmonic, Isothermal, Pulse), which are also plain
+    callables ``f(x, t)``; the force is evaluated at the clock ``t0``, which
+    advances by dt/nleap after every force evaluation, as in the reference.
+    Other callables raise NotImplementedError at the first ``next()``.
+    ``approx=False`` (the exact event-driven solver) is not provided by this
+    library: the first ``next()`` raises NotImplementedError.  ``maxcoll``
+    and ``warn_maxcoll`` only matter to it and are accepted for signature
+    compatibility.
     """
     if approx:
         if nleap is None:
@@ -155,8 +175,11 @@ def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
 def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
                   sort='quick', twopiG=1., full_output=False):
     """Approx path (wendy/wendy.py:336-437) with device-resident state."""
-    if ext_force is not None:
-        raise NotImplementedError('ext_force is not supported on the GPU path')
+    if ext_force is not None and not isinstance(ext_force, ExtForce):
+        raise NotImplementedError(
+            'ext_force must be one of wendy_amd.forces.Harmonic, Isothermal, '
+            'Pulse (evaluated on the GPU); arbitrary Python callables are not '
+            'supported')
     _sort_type_dict[sort]  # unknown names raise KeyError, as the reference
     if omega is None:
         omega2 = -1.
@@ -170,6 +193,8 @@ def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
     m = numpy.require(m, dtype=numpy.float64, requirements=['C', 'W'])
     totmass = numpy.sum(m)
     sim = Sim(x, v, m, totmass, omega2)
+    if ext_force is not None:
+        sim.set_ext_force(ext_force, t0)
     dt_leap = dt / nleap
     try:
         while True:
