@@ -237,6 +237,31 @@ int wdy_pool_stats(wdy_pool *pool, int *live, int64_t *used_sheets,
                    int64_t *largest_free);
 int wdy_pool_destroy(wdy_pool *pool);
 
+/* ---- exact event-driven solver (CPU) ---------------------------------------
+ * The reference's approx=False path (wendy/wendy.c:33-315 with the set-up of
+ * wendy/wendy.py:158-279): sheets fly on parabolas (or, with omega > 0, on
+ * harmonic orbits about their equilibrium offsets) between crossings of
+ * position neighbours; each crossing swaps the pair and updates the forces.
+ * Runs on the host, as a machine-precision check of the GPU leapfrog.
+ *   m       masses already multiplied by twopiG and, with omega > 0, divided
+ *           by omega**2 (wendy/wendy.py:170-174); all > 0
+ *   omega   external harmonic frequency, or <= 0 for none
+ * wdy_exact_advance runs one output interval dt; *ncoll receives its number
+ * of crossings (the reference's ncoll_c) and *hit_max is set when maxcoll
+ * were processed (the reference's err = -2), in which case the remaining
+ * crossings of the interval were skipped, as in the reference;
+ * wdy_exact_reset then re-derives order, forces and crossing times from the
+ * current positions (the warn_maxcoll recovery of wendy/wendy.py:225-232).
+ * Ties in x at set-up keep the lower sheet index first. */
+typedef struct wdy_exact wdy_exact;
+int wdy_exact_create(int64_t n, const double *x, const double *v, const double *m,
+                     double omega, wdy_exact **out);
+int wdy_exact_advance(wdy_exact *ex, double dt, int64_t maxcoll, int64_t *ncoll,
+                      int *hit_max, double *seconds_out);
+int wdy_exact_reset(wdy_exact *ex);
+int wdy_exact_read(wdy_exact *ex, double *x_out, double *v_out);
+int wdy_exact_destroy(wdy_exact *ex);
+
 /* ---- reference-signature compatibility entry point ----------------------
  * Same name, arguments and meaning as the reference's
  * _wendy_nbody_approx_onestep (wendy/wendy.c:385-393, wendy/wendy.h:29-34),

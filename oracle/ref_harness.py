@@ -112,3 +112,84 @@ class RefApprox:
             self.sort_type, ctypes.byref(self.err),
             ctypes.byref(self.time_elapsed), self.cumulmass)
         return self.x, self.v, self.time_elapsed.value
+
+
+class RefExact:
+    """The compiled reference's exact solver.
+
+    Set-up restated from wendy/wendy.py:166-174 and :243-279 (argsort,
+    running mass sums, crossing times through the library's own
+    _solve_coll_quad/_solve_coll_harm); each ``onestep()`` is one
+    ``_wendy_nbody_onestep`` / ``_wendy_nbody_harm_onestep`` call
+    (wendy/wendy.c:53-315).
+    """
+
+    def __init__(self, x, v, m, dt, omega=None, twopiG=1., maxcoll=100000,
+                 variant=''):
+        lib = load(variant)
+        self.lib = lib
+        i32 = ndpointer(dtype=numpy.int32, flags=('C_CONTIGUOUS', 'WRITEABLE'))
+        base = [ctypes.c_int, _F64, _F64, _F64, _F64, i32,
+                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                _F64, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+        lib._wendy_nbody_onestep.restype = None
+        lib._wendy_nbody_onestep.argtypes = base
+        lib._wendy_nbody_harm_onestep.restype = None
+        lib._wendy_nbody_harm_onestep.argtypes = base + [ctypes.c_double]
+        lib._solve_coll_quad.restype = ctypes.c_double
+        lib._solve_coll_quad.argtypes = [ctypes.c_double] * 3
+        lib._solve_coll_harm.restype = ctypes.c_double
+        lib._solve_coll_harm.argtypes = [ctypes.c_double] * 4
+        self.omega = omega
+        self.dt = dt
+        self.maxcoll = maxcoll
+        self.x = numpy.require(copy.copy(x), dtype=numpy.float64, requirements=['C', 'W'])
+        self.v = numpy.require(copy.copy(v), dtype=numpy.float64, requirements=['C', 'W'])
+        self.m = numpy.require(twopiG * copy.copy(m), dtype=numpy.float64,
+                               requirements=['C', 'W'])
+        if omega is not None:
+            self.m /= omega ** 2.
+        self.setup()
+        self.ncoll = 0
+
+    def setup(self):
+        x, v, m = self.x, self.v, self.m
+        sindx = numpy.argsort(x)
+        below = numpy.cumsum(m[sindx])
+        below[-1] = 0.
+        below = numpy.roll(below, 1)
+        above = numpy.cumsum(m[sindx][::-1])[::-1]
+        above[0] = 0.
+        above = numpy.roll(above, -1)
+        a = (above - below)[numpy.argsort(sindx)]
+        xs, vs, as_ = x[sindx], v[sindx], a[sindx]
+        tc = []
+        for k in range(len(x) - 1):
+            if self.omega is None:
+                tc.append(self.lib._solve_coll_quad(xs[k] - xs[k + 1], vs[k] - vs[k + 1],
+                                                    (as_[k] - as_[k + 1]) / 2.))
+            else:
+                tc.append(self.lib._solve_coll_harm(xs[k] - xs[k + 1], vs[k] - vs[k + 1],
+                                                    as_[k] - as_[k + 1], self.omega))
+        self.tcoll = numpy.require(numpy.array(tc), dtype=numpy.float64,
+                                   requirements=['C', 'W'])
+        self.a = numpy.require(a, dtype=numpy.float64, requirements=['C', 'W'])
+        self.sindx = numpy.require(sindx, dtype=numpy.int32, requirements=['C', 'W'])
+        self.cindx = ctypes.c_int(int(numpy.argmin(self.tcoll)))
+        self.next_tcoll = ctypes.c_double(self.tcoll[self.cindx.value])
+        self.err = ctypes.c_int(0)
+
+    def onestep(self):
+        nc = ctypes.c_int(0)
+        te = ctypes.c_double(0.)
+        args = [len(self.x), self.x, self.v, self.a, self.m, self.sindx,
+                ctypes.byref(self.cindx), ctypes.byref(self.next_tcoll), self.tcoll,
+                self.dt, self.maxcoll, ctypes.byref(self.err), ctypes.byref(nc),
+                ctypes.byref(te)]
+        if self.omega is None:
+            self.lib._wendy_nbody_onestep(*args)
+        else:
+            self.lib._wendy_nbody_harm_onestep(*args, self.omega)
+        self.ncoll += nc.value
+        return self.x, self.v, self.ncoll, self.err.value

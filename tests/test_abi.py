@@ -91,10 +91,8 @@ def test_unknown_sort_raises_keyerror_lazily():
         next(g)
 
 
-def test_exact_solver_and_ext_force_not_provided():
-    g = wendy_amd.nbody(numpy.zeros(2), numpy.zeros(2), numpy.ones(2), 0.1)
-    with pytest.raises(NotImplementedError):
-        next(g)
+def test_python_callable_ext_force_not_provided():
+    """Only the device-evaluated forms of wendy_amd.forces are accepted."""
     g = wendy_amd.nbody(numpy.zeros(2), numpy.zeros(2), numpy.ones(2), 0.1,
                         approx=True, nleap=2, ext_force=lambda x, t: 0.)
     with pytest.raises(NotImplementedError):

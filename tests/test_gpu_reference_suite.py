@@ -239,3 +239,18 @@ def test_compat_entry_point_matches_reference_state(gpu_lib, n):
     a_sorted = O.force_sorted(val_force, ms, numpy.sum(m), omega2)
     assert rel_inf(a[xi['idx']], a_sorted) <= 1e-9
     assert rel_inf(cum, O.exclusive_mass_prefix(ms)) <= 1e-12
+
+
+@pytest.mark.parametrize('omega', [None, 1.1])
+def test_approx_against_exact(gpu_lib, omega):
+    """tests/test_approx.py:213-232 and tests/test_approx_harm.py's
+    counterpart: the GPU leapfrog (nleap = 2000) and the exact event-driven
+    solver (CPU) agree to 1e-5 over 100 outputs."""
+    x, v, m = _disk(numpy.random.RandomState(11))
+    g = wendy_amd.nbody(x, v, m, 0.05, approx=True, nleap=2000, omega=omega)
+    ge = wendy_amd.nbody(x, v, m, 0.05, omega=omega)
+    for _ in range(100):
+        tx, tv = next(g)
+        txe, tve = next(ge)
+        assert numpy.all(numpy.fabs(tx - txe) < 1e-5)
+        assert numpy.all(numpy.fabs(tv - tve) < 1e-5)

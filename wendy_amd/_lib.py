@@ -36,6 +36,8 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
             'wdy_set_ext_force', 'wdy_time',
+            'wdy_exact_create', 'wdy_exact_advance', 'wdy_exact_reset',
+            'wdy_exact_read', 'wdy_exact_destroy',
             'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
             'wdy_batch_destroy',
             'wdy_pool_create', 'wdy_pool_add', 'wdy_pool_step', 'wdy_pool_read',
@@ -108,6 +110,19 @@ def _declare(lib):
     lib.wdy_batch_read.argtypes = [S, S, S, _c_double_p]
     lib.wdy_batch_destroy.restype = ctypes.c_int
     lib.wdy_batch_destroy.argtypes = [S]
+    lib.wdy_exact_create.restype = ctypes.c_int
+    lib.wdy_exact_create.argtypes = [ctypes.c_int64, _F64, _F64, _F64, ctypes.c_double,
+                                     ctypes.POINTER(S)]
+    lib.wdy_exact_advance.restype = ctypes.c_int
+    lib.wdy_exact_advance.argtypes = [S, ctypes.c_double, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int), _c_double_p]
+    lib.wdy_exact_reset.restype = ctypes.c_int
+    lib.wdy_exact_reset.argtypes = [S]
+    lib.wdy_exact_read.restype = ctypes.c_int
+    lib.wdy_exact_read.argtypes = [S, S, S]
+    lib.wdy_exact_destroy.restype = ctypes.c_int
+    lib.wdy_exact_destroy.argtypes = [S]
     lib.wdy_set_ext_force.restype = ctypes.c_int
     lib.wdy_set_ext_force.argtypes = [S, ctypes.c_int, _F64, ctypes.c_double]
     lib.wdy_time.restype = ctypes.c_int

@@ -33,9 +33,15 @@ def hipcc():
     return os.environ.get('HIPCC', shutil.which('hipcc') or '/opt/rocm/bin/hipcc')
 
 
+HOST_FLAGS = ['-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', '-Wall',
+              '-I', os.path.join(ROOT, 'include')]
+
+
 def sources():
+    """HIP sources (device + host code, built by hipcc for gfx950) and plain
+    C++ host sources (built by g++)."""
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
-                  if f.endswith('.hip'))
+                  if f.endswith('.hip') or f.endswith('.cpp'))
 
 
 def build(verbose=False):
@@ -44,9 +50,13 @@ def build(verbose=False):
     procs = []
     objs = []
     for src in sources():
-        obj = os.path.join(BUILD, os.path.basename(src)[:-4] + '.o')
+        base, ext = os.path.splitext(os.path.basename(src))
+        obj = os.path.join(BUILD, base + ('.o' if ext == '.hip' else '.host.o'))
         objs.append(obj)
-        cmd = [cc, '--offload-arch=' + ARCH, *COMMON, '-c', src, '-o', obj]
+        if ext == '.hip':
+            cmd = [cc, '--offload-arch=' + ARCH, *COMMON, '-c', src, '-o', obj]
+        else:
+            cmd = [os.environ.get('CXX', 'g++'), *HOST_FLAGS, '-c', src, '-o', obj]
         if verbose:
             print(' '.join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,

@@ -9,6 +9,7 @@ positions and velocities back into the generator's own ``x``, ``v`` arrays.
 """
 import copy
 import ctypes
+import warnings
 
 import numpy
 
@@ -154,10 +155,10 @@ def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
     callables ``f(x, t)``; the force is evaluated at the clock ``t0``, which
     advances by dt/nleap after every force evaluation, as in the reference.
     Other callables raise NotImplementedError at the first ``next()``.
-    ``approx=False`` (the exact event-driven solver) is not provided by this
-    library: the first ``next()`` raises NotImplementedError.  ``maxcoll``
-    and ``warn_maxcoll`` only matter to it and are accepted for signature
-    compatibility.
+    ``approx=False`` runs the exact event-driven solver on the CPU
+    (libwendy_amd's wdy_exact_*), the reference's own machine-precision
+    method, with its checks and its ``maxcoll``/``warn_maxcoll`` behaviour;
+    ``full_output`` then yields ``(x, v, ncoll, seconds)``.
     """
     if approx:
         if nleap is None:
@@ -167,9 +168,61 @@ def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
                                   omega=omega, ext_force=ext_force,
                                   twopiG=twopiG, full_output=full_output):
             yield item
-    raise NotImplementedError(
-        'wendy_amd implements the approximate integration path only '
-        '(approx=True); the exact event-driven solver is not provided')
+    for item in _nbody_exact(x, v, m, dt, twopiG=twopiG, omega=omega,
+                             ext_force=ext_force, maxcoll=maxcoll,
+                             warn_maxcoll=warn_maxcoll, full_output=full_output):
+        yield item
+
+
+def _nbody_exact(x, v, m, dt, twopiG=1., omega=None, ext_force=None,
+                 maxcoll=100000, warn_maxcoll=False, full_output=False):
+    """Exact path (wendy/wendy.py:158-238) on the CPU: same checks and
+    messages, same masses (x twopiG, / omega**2), same maxcoll handling."""
+    if ext_force is not None:
+        raise ValueError('External forces (ext_force=) are not supported for '
+                         'the exact nbody calculation')
+    if omega is not None and omega * dt > numpy.pi / 2.:
+        raise ValueError('When omega is set, omega*dt needs to be less than '
+                         'pi/2; please adjust dt')
+    if numpy.any(m < numpy.median(m) * 10. ** -8.):
+        raise ValueError('Tiny masses m much smaller than the median mass are '
+                         'not supported, please remove these from the inputs')
+    x = numpy.require(copy.copy(x), dtype=numpy.float64, requirements=['C', 'W'])
+    v = numpy.require(copy.copy(v), dtype=numpy.float64, requirements=['C', 'W'])
+    m = numpy.require(twopiG * copy.copy(m), dtype=numpy.float64,
+                      requirements=['C', 'W'])
+    if omega is not None:
+        m /= omega ** 2.
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    _lib.check(lib.wdy_exact_create(len(x), x, v, m,
+                                    -1. if omega is None else float(omega),
+                                    ctypes.byref(h)))
+    ncoll = 0
+    c = ctypes.c_int64(0)
+    hit = ctypes.c_int(0)
+    sec = ctypes.c_double(0.)
+    try:
+        while True:
+            _lib.check(lib.wdy_exact_advance(h, float(dt), int(maxcoll),
+                                             ctypes.byref(c), ctypes.byref(hit),
+                                             ctypes.byref(sec)))
+            ncoll += c.value
+            _lib.check(lib.wdy_exact_read(h, x.ctypes.data_as(ctypes.c_void_p),
+                                          v.ctypes.data_as(ctypes.c_void_p)))
+            if hit.value:
+                if not warn_maxcoll:
+                    raise RuntimeError('Maximum number of collisions per time '
+                                       'step exceeded')
+                warnings.warn('Maximum number of collisions per time step '
+                              'exceeded', RuntimeWarning)
+                _lib.check(lib.wdy_exact_reset(h))
+            if full_output:
+                yield (x, v, ncoll, sec.value)
+            else:
+                yield (x, v)
+    finally:
+        lib.wdy_exact_destroy(h)
 
 
 def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
