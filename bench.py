@@ -1,6 +1,7 @@
 """Benchmark of the approximate-integration path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--workload plain24|unequal27] [--no-ensemble] [--no-serve]
 
 Workload (the single-GPU drop-in configuration): N = 2^24 equal-mass sheets
 (m = 2^-24, twopiG = 1) drawn from an isothermal sech^2 disk (z_h = 1,
@@ -12,6 +13,15 @@ timed region; the timed region is K output intervals of device work bracketed
 by stream synchronisations.  ``value`` is N * nleap * K / seconds
 (FP64 sheet-steps/s).  The end-to-end rate through the generator (with the
 per-output de-sort and device-to-host copy of x and v) is reported beside it.
+``--workload unequal27`` measures the large unequal-mass disk instead (N =
+2^27, lognormal(0, 0.5) masses, +0.3 bulk velocity on x > 0, dt = 0.005,
+nleap = 10) -- not the headline line.
+
+Beside the headline line the default run reports, in the same JSON object,
+``ensemble`` (4096 disks of N log-uniform in [2^8, 2^16] advanced together
+in one launch per output interval, against looping the plain path over a
+sample of the same disks) and ``serving`` (a bounded window of the Poisson
+request load through wendy_amd.serve, p50/p99 completion latency).
 
 Roofline: the dominant kernel is timed with HIP events on the simulation's
 stream during the timed region (per-launch average); ``achieved`` is its
@@ -40,8 +50,10 @@ HBM_PEAK = 8.0e12
 KC_SORT_HIST, KC_SORT_PASS, KC_LEAP, KC_SCAN, KC_OTHER, KC_WARM = range(6)
 
 
-def workload(n, seed=0):
-    """sech^2 disk, z_h = sigma = 1, equal masses 1/n (RandomState(seed))."""
+def workload(n, seed=0, lognormal=None, vshift=None):
+    """sech^2 disk, z_h = sigma = 1, from RandomState(seed) drawing u, then
+    v, then masses: equal 1/n, or lognormal(0, s) normalised to 1; vshift
+    adds a bulk velocity to the sheets with x > 0."""
     rs = numpy.random.RandomState(seed)
     u = rs.uniform(size=n)
     bad = u == 0.
@@ -50,8 +62,29 @@ def workload(n, seed=0):
         bad = u == 0.
     x = 2. * numpy.arctanh(2. * u - 1.)
     v = rs.normal(size=n)
-    m = numpy.full(n, 1. / n)
+    if lognormal is None:
+        m = numpy.full(n, 1. / n)
+    else:
+        m = rs.lognormal(0., lognormal, size=n)
+        m /= m.sum()
+    if vshift is not None:
+        v[x > 0] += vshift
     return x, v, m
+
+
+WORKLOADS = {
+    'plain24': dict(n=1 << 24, nleap=20, dt=0.005, omega=0.1, lognormal=None,
+                    vshift=None,
+                    label=('plain single-simulation drop-in: N=2^24 equal-mass '
+                           'sech^2 disk (z_h=1, sigma=1), omega=0.1, dt=0.005, '
+                           'nleap=20; one step = one output interval')),
+    'unequal27': dict(n=1 << 27, nleap=10, dt=0.005, omega=None, lognormal=0.5,
+                      vshift=0.3,
+                      label=('large unequal-mass disk: N=2^27, lognormal(0,0.5) '
+                             'masses normalised to 1, v+=0.3 for x>0, no omega, '
+                             'dt=0.005, nleap=10; one step = one output '
+                             'interval')),
+}
 
 
 def disk(n, rs, zh=1., sigma=1., lognormal=None):
@@ -285,10 +318,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--n', type=int, default=1 << 24)
-    ap.add_argument('--nleap', type=int, default=20)
-    ap.add_argument('--dt', type=float, default=0.005)
-    ap.add_argument('--omega', type=float, default=0.1)
+    ap.add_argument('--workload', choices=sorted(WORKLOADS), default='plain24')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--no-ensemble', action='store_true')
@@ -311,13 +341,15 @@ def main():
         copy_probe()
         return
     import wendy_amd
-    n, nleap, dt, omega = args.n, args.nleap, args.dt, args.omega
-    x, v, m = workload(n)
+    wl = WORKLOADS[args.workload]
+    n, nleap, dt, omega = wl['n'], wl['nleap'], wl['dt'], wl['omega']
+    x, v, m = workload(n, lognormal=wl['lognormal'], vshift=wl['vshift'])
+    equal = wl['lognormal'] is None
     h = dt / nleap
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     log('bench: n=%d nleap=%d steps=%d warmup=%d' % (n, nleap, args.steps,
                                                      args.warmup))
-    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), omega ** 2.)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.)
     for _ in range(args.warmup):
         sim.advance(h, nleap)
     sim.read()
@@ -356,8 +388,10 @@ def main():
         name, bpl = 'radix scatter pass', 24 * n
         note = 'read key 8 B + payload 4 B, write key 8 B + payload 4 B per sheet'
     elif dom == KC_WARM:
-        name, bpl = 'warm-start bucket sort + leap', 40 * n
-        note = 'read key, v, idx; write key, v, idx (40 B per sheet)'
+        per = 40 if equal else 56
+        name, bpl = 'warm-start bucket sort + leap', per * n
+        note = ('read key, v, idx%s; write the same (%d B per sheet)'
+                % ('' if equal else ', mass', per))
     elif dom == KC_LEAP:
         name, bpl = 'fused force+kick+drift', 44 * n
         note = 'read key, perm, v, idx; write key, v, idx'
@@ -367,7 +401,7 @@ def main():
     avg_s = ms * 1e-3 / max(cnt, 1)
     achieved = bpl / avg_s if avg_s > 0 else 0.
     traffic, traffic_src = None, None
-    if dom == KC_WARM and n == 1 << 24 and nleap == 20:
+    if dom == KC_WARM and args.workload == 'plain24':
         traffic, traffic_src = measured_traffic('warm_step_n16777216_hbm_traffic.md')
     roofline = {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9,
                 'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': traffic,
@@ -375,14 +409,18 @@ def main():
                 'kernel': name, 'bytes_per_launch': bpl,
                 'avg_launch_us': avg_s * 1e6, 'launches': cnt,
                 'bytes_note': note,
-                'step_frac_B236': rate * 236 / HBM_PEAK,
-                'step_frac_B40': rate * 40 / HBM_PEAK}
+                # whole-step rate against the full-radix-resort model
+                # (236 / 272 B per sheet-step) and the compulsory state
+                # traffic (40 / 48 B), equal / unequal masses
+                'step_frac_Balg': rate * (236 if equal else 272) / HBM_PEAK,
+                'step_frac_Bmin': rate * (40 if equal else 48) / HBM_PEAK}
 
-    ens = None if args.no_ensemble else bench_ensemble(args, log)
-    srv = None if args.no_serve else bench_serve(args, log)
+    main_line = args.workload == 'plain24'
+    ens = None if args.no_ensemble or not main_line else bench_ensemble(args, log)
+    srv = None if args.no_serve or not main_line else bench_serve(args, log)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and main_line:
         log('bench: cpu baseline ...')
         cpu = cpu_baseline(x, v, m, dt, nleap, omega, args.cpu_threads)
         if cpu:
@@ -394,11 +432,8 @@ def main():
         'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic',
-        'config': {'workload': ('plain single-simulation drop-in: N=2^24 '
-                                'equal-mass sech^2 disk (z_h=1, sigma=1), '
-                                'omega=0.1, dt=0.005, nleap=20; one step = '
-                                'one output interval'),
-                   'n': n, 'nleap': nleap, 'dt': dt, 'omega': omega},
+        'config': {'workload': wl['label'], 'n': n, 'nleap': nleap, 'dt': dt,
+                   'omega': omega},
         'e2e_value': e2e,
         'kernel_ms': {str(c): kt[c][0] for c in kt},
         'roofline': roofline,

@@ -110,7 +110,7 @@ def test_warm_capacity_limit_bucket(gpu_lib):
     """Buckets targeted at the LDS capacity: any that outgrows it fails the
     interval, which is redone; results are unchanged either way."""
     x, v, m = make(1 << 17, 4)
-    warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, tune=(4096, 8))
+    warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, tune=(4096, 8, 0))
     assert warm + cold == 20
 
 
@@ -180,3 +180,19 @@ def test_warm_window_limit_per_shape(gpu_lib):
         sim.tune(0, 64, 0)
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize('mass', ['equal', 'lognormal'])
+@pytest.mark.parametrize('pipe', ['0', '1'])
+def test_warm_kernel_forms_agree(gpu_lib, monkeypatch, mass, pipe):
+    """The one-workgroup-per-bucket kernel (WDY_WARM_PIPE=0) and the
+    persistent pipelined kernel (the default) both give the oracle's
+    trajectory, including through a redo and small buckets."""
+    monkeypatch.setenv('WDY_WARM_PIPE', pipe)
+    print('pipe', pipe, mass, flush=True)
+    x, v, m = make(50000, 21, mass)
+    warm, cold, redo = run_both(x, v, m, 0.01, 5, 3, omega=0.1)
+    assert warm == 10 and redo == 0
+    x, v, m = make(9000, 22, mass)
+    warm, _, _ = run_both(x, v, m, 0.02, 4, 3, tune=(64, 2))
+    assert warm > 0

@@ -1,4 +1,5 @@
 // C ABI of libwendy_amd.so (declared in include/wendy_amd.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -138,6 +139,31 @@ static void free_sim(wdy_sim *s) {
   delete s;
 }
 
+// Ranks per unit time crossed by the fastest sheets: n * (peak density of a
+// strided sample of up to 4096 positions, 32-neighbour estimate) * (largest
+// |v| of the sample).  A sizing hint only.
+static double displacement_rate(int64_t n, const double *x_slot, const double *v_id,
+                                const i32 *id_slot) {
+  const int64_t ns = n < 4096 ? n : 4096, stride = n / ns;
+  if (ns < 64) return 0.0;
+  std::vector<double> xs((size_t)ns);
+  double vmax = 0.0;
+  for (int64_t i = 0; i < ns; ++i) {
+    const int64_t k = i * stride;
+    xs[(size_t)i] = x_slot[k];
+    const double vk = std::fabs(v_id[id_slot ? id_slot[k] : k]);
+    vmax = vk > vmax ? vk : vmax;
+  }
+  std::sort(xs.begin(), xs.end());
+  const int64_t K = 32;
+  double rho = 0.0;
+  for (int64_t i = 0; i + K < ns; ++i) {
+    const double w = xs[(size_t)(i + K)] - xs[(size_t)i];
+    if (w > 0.0) rho = std::max(rho, (double)K / ((double)ns * w));
+  }
+  return (double)n * rho * vmax;
+}
+
 // Creates a sim from positions given per slot (x_slot[i] is the position of
 // sheet id_slot[i]; id_slot null means the identity) and velocities/masses
 // given per sheet (v_id[j], m_id[j]).  The positions are sorted on the device
@@ -236,8 +262,9 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     bool warm = !s->small && !(flags & WDY_NO_WARM);
     for (int64_t i = 0; warm && i < n; ++i) warm = m_id[i] >= 0.0;
     if (warm) {
-      r = warm_configure(s, WARM_SHAPE_DEFAULT, 0u, 8u);
+      r = warm_configure(s, eq ? WARM_SHAPE_EQ : WARM_SHAPE_UNEQ, 0u, 8u);
       if (r) return r;
+      s->disp_rate = displacement_rate(n, x_slot, v_id, id_slot);
     }
     WDY_HIP(hipStreamSynchronize(s->stream));
     return WDY_OK;
@@ -289,6 +316,8 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
     if (s->ext.kind) s->t0 = t;
     s->cold_steps += nleap;
     if (s->warm) {
+      rc = warm_fit(s, h);
+      if (rc) return rc;
       rc = warm_partition_after_cold(s, h / 2.);
       if (rc) return rc;
     }
@@ -389,6 +418,7 @@ int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {
                 bucket_target, window);
   WDY_HIP(hipSetDevice(sim->device));
   WDY_HIP(hipStreamSynchronize(sim->stream));
+  warm_mark_tuned(sim);
   return warm_configure(sim, shape, (u32)bucket_target, (u32)window);
 }
 

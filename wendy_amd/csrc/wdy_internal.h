@@ -178,6 +178,9 @@ struct wdy_sim {
   // evaluation while an external force is set (wendy/wendy.c:401-402,407-408)
   wdy::ExtForce ext;
   double t0 = 0.0;
+  // ranks the fastest sheets cross per unit time, estimated from a sample
+  // at creation (sizes the warm-start window once h is known)
+  double disp_rate = 0.0;
 
   int64_t warm_steps = 0, cold_steps = 0, redos = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -218,6 +221,11 @@ void warm_free(wdy_sim *s);
 bool warm_ready_for(wdy_sim *s, double h);
 u64 *warm_cold_splitters(wdy_sim *s, u32 *T, u32 *nb);
 int warm_partition_after_cold(wdy_sim *s, double hp);
+// Before the first warm interval: pick the window (and, if it would exceed
+// the shape's limit, the largest buckets) from s->disp_rate * h.  Does
+// nothing after wdy_tune or once done.
+int warm_fit(wdy_sim *s, double h);
+void warm_mark_tuned(wdy_sim *s);
 int warm_interval(wdy_sim *s, double h, int nleap, bool *failed);
 int warm_current_shape(wdy_sim *s);
 int launch_energy_arrays(int64_t n, const double *x, const double *v,

@@ -18,8 +18,12 @@ struct WarmShape {
     return w < WARM_WMAX ? w : WARM_WMAX;
   }
 };
-constexpr int WARM_NSHAPES = 7;
-constexpr int WARM_SHAPE_DEFAULT = 0;
+constexpr int WARM_NSHAPES = 9;
+// default tile shapes (measured at N = 2^24 with the pipelined kernel):
+// 256 threads per workgroup, several workgroups per CU
+constexpr int WARM_SHAPE_EQ = 8;     // equal masses: 1536-sheet buckets
+constexpr int WARM_SHAPE_UNEQ = 2;   // unequal masses: 1024-sheet buckets
+constexpr int WARM_SHAPE_DEFAULT = WARM_SHAPE_UNEQ;
 const WarmShape &warm_shape(int v);
 constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry
 constexpr u32 WARM_T_MIN = 32;
@@ -71,6 +75,8 @@ constexpr u32 WARM_ERR_OVF = 2;       // the overflow list was full
 constexpr u32 WARM_ERR_LOVF = 4;      // too many far movers into one bucket
 constexpr u32 WARM_ERR_STALL = 8;     // look-back gave up (bounded wait)
 
+// mode 0: one workgroup per bucket; 1: partition a radix-sorted array;
+// 2: mode 0 by persistent workgroups that prefetch the next bucket
 int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
                      bool harm, int mode, int shape);
 // zero the ticket ring, the overflow-count ring except slot `keep` (4: none),

@@ -42,6 +42,7 @@
 // exceeds its bounded wait) records the failing launch's epoch in err[0];
 // later launches exit at once, and the host redoes the interval from its
 // untouched start state on the radix path.
+#include <cmath>
 #include <cstdlib>
 #include <vector>
 
@@ -472,6 +473,379 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
   WARM_STAMP(5);
 }
 
+// Persistent, software-pipelined form of mode 0.  One workgroup per CU loops
+// over tickets; while it sorts, looks back and writes bucket j, the loads of
+// the next bucket are in flight: its segment table and splitter window are
+// requested right after bucket j is committed to LDS, and its sheets right
+// after j's look-back, so they arrive during j's force and partition.  The
+// arithmetic and the outputs are those of k_warm_step.  Tables are
+// double-buffered in LDS; each workgroup holds at most two unpublished
+// tickets, and the lowest unpublished ticket always belongs to a workgroup
+// whose own look-back waits only on lower, published buckets, so the chain
+// always advances.
+template <class SH, bool EQ, bool HARM>
+__global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
+  constexpr int WT = SH::WT, PER = SH::PER, CAP = SH::CAP, NWAVE = SH::NWAVE;
+  extern __shared__ u64 lds_dyn[];
+  Lds<SH, EQ> s(lds_dyn);
+  __shared__ u64 s_spl[2][SMAX + 1];
+  __shared__ u32 s_segsrc[2][SMAX];
+  __shared__ u32 s_segpre[2][SMAX + 1];
+  __shared__ u32 s_oidx[2][LOVF];
+  __shared__ u32 s_novf[2];
+  __shared__ u64 s_red[NWAVE];
+  __shared__ u64 s_mm[2 * NWAVE];
+  __shared__ u32 s_u[4];
+  __shared__ u64 s_u64[2];
+
+  const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const u32 S = A.S, W = A.W, nb = A.nb;
+  if (t == 0) {
+    s_u[0] = atomicAdd(A.tick, 1u);
+    s_u[1] = ld_u32(A.err);
+    s_novf[0] = 0;
+    s_novf[1] = 0;
+  }
+  const u32 novf_in = ld_u32(A.ovf_r);
+  const u32 novf_all = novf_in > WARM_OVF_CAP ? WARM_OVF_CAP : novf_in;
+  __syncthreads();
+  if (s_u[1] != 0 && s_u[1] < A.epoch) return;  // an earlier launch failed
+  u32 j = s_u[0];
+  if (j == 0 && t == 0) {  // clear the slots launch epoch+2 will use
+    *A.tick_next2 = 0;
+    *A.ovf_next2 = 0;
+  }
+  const u64 ep = (u64)(A.epoch & 0x3FFFFFFFu) << 34;
+
+  // table registers of the bucket being prefetched
+  u32 tb_cnt = 0, tb_start = 0;
+  u64 tb_spl = 0;
+  // requests bucket q's segment table and splitter window into registers
+  auto table_issue = [&](u32 q) {
+    const u32 lo = q >= W ? q - W : 0u, hi = q + W < nb ? q + W : nb - 1;
+    tb_cnt = tb_start = 0;
+    if (t < S) {
+      const int src = (int)q - (int)t + (int)W;
+      if (src >= 0 && src < (int)nb) {
+        tb_cnt = A.rd.segcnt[(u32)src * (S + 1) + t];
+        tb_start = A.rd.boff[src] + A.rd.segoff[(u32)src * (S + 1) + t];
+      }
+    }
+    if (t <= hi - lo + 1) tb_spl = A.rd.spl[lo + t];
+  };
+  // stores the registers into buffer bb, collects q's overflow entries and
+  // scans the segment counts; returns the segment sheets
+  auto table_commit = [&](u32 q, int bb) -> u32 {
+    const u32 lo = q >= W ? q - W : 0u, hi = q + W < nb ? q + W : nb - 1;
+    if (t < S) {
+      s_segsrc[bb][t] = tb_start;
+      s_segpre[bb][t] = tb_cnt;
+    }
+    if (t <= hi - lo + 1) s_spl[bb][t] = tb_spl;
+    for (u32 q2 = t; q2 < novf_all; q2 += WT)
+      if (A.rd.odst[q2] == q) {
+        const u32 slot = atomicAdd(&s_novf[bb], 1u);
+        if (slot < (u32)LOVF) s_oidx[bb][slot] = q2;
+      }
+    __syncthreads();
+    return block_excl_u32_inplace<NWAVE>(s_segpre[bb], S, s_red);
+  };
+
+  // data registers of the bucket being prefetched
+  u64 dk[PER];
+  double dv[PER], dm[PER];
+  i32 di[PER];
+  auto data_issue = [&](u32 nseg, u32 c, int bb) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const u32 e = i * WT + t;
+      if (e < c) {
+        const bool seg = e < nseg;
+        const u32 pos = seg ? seg_pos(s_segpre[bb], s_segsrc[bb], S, e)
+                            : s_oidx[bb][e - nseg];
+        dk[i] = (seg ? A.kin : A.rd.okey)[pos];
+        dv[i] = (seg ? A.vin : A.rd.ov)[pos];
+        di[i] = (seg ? A.iin : A.rd.oid)[pos];
+        if (!EQ) dm[i] = (seg ? A.min : A.rd.om)[pos];
+      }
+    }
+  };
+
+  int b = 0;
+  u32 nseg = 0, c = 0;
+  bool ok = false;
+  if (j < nb) {  // prologue: bucket j's tables and loads
+    table_issue(j);
+    nseg = table_commit(j, b);
+    const u32 novf = s_novf[b];
+    c = nseg + novf;
+    ok = c <= (u32)CAP && novf <= (u32)LOVF;
+    if (ok) data_issue(nseg, c, b);
+  }
+
+  while (j < nb) {
+    const u32 lo = j >= W ? j - W : 0u, hi = j + W < nb ? j + W : nb - 1;
+    // ---- commit bucket j to LDS; publish its aggregate ----------------------
+    if (t == 0) s_u[0] = atomicAdd(A.tick, 1u);  // the next ticket
+    u64 msum = 0;
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 e = i * WT + t;
+        if (e < c) {
+          u64 kk = dk[i];
+          if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * dv[i]);
+          s.key[e] = kk;
+          s.v[e] = dv[i];
+          s.id[e] = di[i];
+          if (!EQ) {
+            s.m[e] = dm[i];
+            msum += mass_fix(dm[i], A.fscale);
+          }
+        }
+      }
+    } else if (!EQ) {
+      // over capacity: still publish the exact mass total
+      for (u32 e = t; e < nseg; e += WT)
+        msum += mass_fix(A.min[seg_pos(s_segpre[b], s_segsrc[b], S, e)], A.fscale);
+      for (u32 q2 = t; q2 < novf_all; q2 += WT)
+        if (A.rd.odst[q2] == j) msum += mass_fix(A.rd.om[q2], A.fscale);
+    }
+    u64 mtot = 0;
+    if (!EQ) (void)block_excl_u64<NWAVE>(msum, s_red, &mtot);
+    if (t == 0) {
+      publish(A.status + STATUS_WORDS * j, ep, j == 0 ? 2u : 1u, c, mtot, !EQ);
+      if (!ok)
+        set_err(A.err, A.epoch,
+                c - nseg > (u32)LOVF ? WARM_ERR_LOVF : WARM_ERR_CAPACITY);
+    }
+    __syncthreads();
+    const u32 jn = s_u[0];
+    const int bn = b ^ 1;
+    if (t == 0) s_novf[bn] = 0;
+    if (jn < nb) table_issue(jn);  // in flight during the sort
+
+    // ---- sort, look-back ---------------------------------------------------------
+    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm, nullptr);
+    else __syncthreads();
+    if (w == 0) {
+      u64 ecnt = 0, emass = 0;
+      bool stalled = false;
+      if (j > 0) {
+        int64_t base = (int64_t)j - 1;
+        u32 spins = 0;
+        while (true) {
+          const int64_t pt = base - (int64_t)lane;
+          u64 wd = 0, mv = 0;
+          u32 flag = 2;
+          if (pt >= 0) {
+            const u64 *st = A.status + STATUS_WORDS * (u64)pt;
+            const u32 e30 = A.epoch & 0x3FFFFFFFu;
+            wd = ld_rlx(st);
+            flag = (u32)(wd >> 34) == e30 ? (u32)(wd >> 32) & 3u : 0u;
+            if (!EQ && flag != 0) {
+              const u64 *mw = st + (flag == 2 ? 3 : 1);
+              const u64 hw = ld_rlx(mw), lw = ld_rlx(mw + 1);
+              const bool good = (u32)(hw >> 32) == ((e30 << 2) | 1u) &&
+                                (u32)(lw >> 32) == ((e30 << 2) | 1u);
+              if (good) mv = ((hw & 0xffffffffull) << 32) | (lw & 0xffffffffull);
+              else flag = 0;
+            }
+          }
+          const u64 incm = __ballot(flag == 2);
+          const u64 zerom = __ballot(flag == 0);
+          const u32 first = incm ? (u32)__ffsll((long long)incm) - 1u : 64u;
+          const u64 upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+          if (zerom & upto) {
+            if (++spins > (1u << 20)) {
+              stalled = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          u64 cc = 0, mm = 0;
+          if (lane <= first && pt >= 0) {
+            cc = (u32)wd;
+            mm = mv;
+          }
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            cc += __shfl_down(cc, o, 64);
+            mm += __shfl_down(mm, o, 64);
+          }
+          ecnt += __shfl(cc, 0, 64);
+          emass += __shfl(mm, 0, 64);
+          if (first < 64) break;
+          base -= 64;
+        }
+        if (lane == 0) {
+          if (stalled) {
+            set_err(A.err, A.epoch, WARM_ERR_STALL);
+          } else {
+            publish(A.status + STATUS_WORDS * j, ep, 2u, (u32)(ecnt + c),
+                    emass + mtot, !EQ);
+          }
+        }
+      }
+      if (lane == 0) {
+        s_u64[0] = ecnt;
+        s_u64[1] = emass;
+        s_u[2] = stalled ? 1u : 0u;
+      }
+    }
+    __syncthreads();
+    const u32 off = (u32)s_u64[0];
+    const u64 bfix = s_u64[1];
+    const bool write = ok && !s_u[2];
+
+    // ---- next bucket's tables to LDS --------------------------------------------
+    u32 nseg_n = 0, c_n = 0;
+    bool ok_n = false;
+    if (jn < nb) {
+      nseg_n = table_commit(jn, bn);
+      const u32 novf_n = s_novf[bn];
+      c_n = nseg_n + novf_n;
+      ok_n = c_n <= (u32)CAP && novf_n <= (u32)LOVF;
+    }
+
+    if (write) {
+      // ---- force, kick, drift in sorted order ------------------------------------
+      {
+        const u32 r0 = t * PER;
+        u64 run = 0;
+        if (!EQ) {
+          u64 part = 0;
+#pragma unroll
+          for (int i = 0; i < PER; ++i)
+            if (r0 + i < c) part += mass_fix(s.m[s.ord[r0 + i]], A.fscale);
+          run = bfix + block_excl_u64<NWAVE>(part, s_red, nullptr);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+          const u32 r = r0 + i;
+          if (r < c) {
+            const u32 e = s.ord[r];
+            const u32 R = off + r;
+            const u64 kold = s.key[e];
+            double x = decode_pos(kold);
+            double F;
+            if (EQ) {
+              F = (A.totmass - 2.0 * ((double)R * A.mq)) - A.mq;
+            } else {
+              const double me = s.m[e];
+              F = (A.totmass - 2.0 * ((double)run * A.finv)) - me;
+              run += mass_fix(me, A.fscale);
+            }
+            if (HARM) F = F - A.omega2 * x;
+            const double a = (A.ext.kind ? ext_accel(A.ext, x, A.t) : 0.0) + F;
+            const double v = s.v[e] + A.h * a;
+            x = x + A.hd * v;
+            s.v[e] = v;
+            s.key[e] = encode_pos(x);
+            if (R % A.T == 0) A.wr.spl[R / A.T] = R == 0 ? 0ull : kold;
+          }
+        }
+        if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
+      }
+      __syncthreads();
+    }
+    // ---- the next bucket's loads, in flight during the partition --------------------
+    if (ok_n) data_issue(nseg_n, c_n, bn);
+    if (write) {
+      // ---- partition by next-step bucket and write out ---------------------------
+      u32 *wc = s.bins;  // [NWAVE][S + 1] per-wave segment counters
+      u32 *segp = s_segpre[b];
+      const u64 *spl = s_spl[b];
+      for (u32 q = t; q < (u32)NWAVE * (S + 1); q += WT) wc[q] = 0;
+      __syncthreads();
+      u32 dd[PER], rr[PER];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 r = i * WT + t;
+        const bool valid = r < c;
+        u32 d = S;
+        if (valid) {
+          const u32 e = s.ord[r];
+          u64 kp = s.key[e];
+          if (A.hp != 0.0) kp = encode_pos(decode_pos(kp) + A.hp * s.v[e]);
+          const bool below = lo > 0 && kp < spl[0];
+          const bool above = hi + 1 < nb && kp >= spl[hi - lo + 1];
+          if (!below && !above) {
+            u32 a = 0, z = hi - lo;
+            while (a < z) {
+              const u32 mid = (a + z + 1) >> 1;
+              if (spl[mid] <= kp) a = mid; else z = mid - 1;
+            }
+            d = lo + a + W - j;
+          } else {
+            const u32 dst = bucket_of(A.rd.spl, nb, kp);
+            const u32 q = atomicAdd(A.ovf_w, 1u);
+            if (q < WARM_OVF_CAP) {
+              A.wr.okey[q] = s.key[e];
+              A.wr.ov[q] = s.v[e];
+              A.wr.oid[q] = s.id[e];
+              A.wr.om[q] = EQ ? A.mq : s.m[e];
+              A.wr.odst[q] = dst;
+            } else {
+              set_err(A.err, A.epoch, WARM_ERR_OVF);
+            }
+          }
+        }
+        u64 peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+          const bool on = (d >> bit) & 1u;
+          const u64 bal = __ballot(on);
+          peers &= on ? bal : ~bal;
+        }
+        const u32 below_l = (u32)__popcll(peers & ((1ull << lane) - 1ull));
+        const u32 old = valid ? wc[w * (S + 1) + d] : 0u;
+        if (valid && below_l == 0) wc[w * (S + 1) + d] = old + (u32)__popcll(peers);
+        dd[i] = d;
+        rr[i] = old + below_l;
+      }
+      __syncthreads();
+      if (t <= S) {
+        u32 tot = 0;
+#pragma unroll
+        for (int i = 0; i < NWAVE; ++i) {
+          const u32 x = wc[i * (S + 1) + t];
+          wc[i * (S + 1) + t] = tot;
+          tot += x;
+        }
+        segp[t] = tot;
+      }
+      __syncthreads();
+      block_excl_u32_inplace<NWAVE>(segp, S + 1, s_red);
+      if (t <= S) {
+        const u32 nxt = t < S ? segp[t + 1] : c;
+        A.wr.segoff[j * (S + 1) + t] = segp[t];
+        A.wr.segcnt[j * (S + 1) + t] = nxt - segp[t];
+      }
+      if (t == 0) A.wr.boff[j] = off;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 r = i * WT + t;
+        if (r < c) {
+          const u32 e = s.ord[r];
+          const u32 p = off + segp[dd[i]] + wc[w * (S + 1) + dd[i]] + rr[i];
+          A.kout[p] = s.key[e];
+          A.vout[p] = s.v[e];
+          A.iout[p] = s.id[e];
+          if (!EQ) A.mout[p] = s.m[e];
+        }
+      }
+    }
+    __syncthreads();  // LDS of bucket j free before the next commit
+    j = jn;
+    b = bn;
+    nseg = nseg_n;
+    c = c_n;
+    ok = ok_n;
+  }
+}
+
 __global__ void k_warm_reset(u32 *ring, u32 keep, u32 *err) {
   const u32 t = threadIdx.x;
   // ring[0..3] tickets, ring[4..7] overflow counts; slot `keep` of the
@@ -494,9 +868,41 @@ int launch_one(hipStream_t st, const WarmArgs &a) {
   return WDY_OK;
 }
 
+// Workgroups of the persistent mode-0 kernel: as many as are resident at
+// once (more would only find the tickets exhausted).
+template <class SH, bool EQ, bool HARM>
+int launch_pipe(hipStream_t st, const WarmArgs &a) {
+  auto fn = k_warm_pipe<SH, EQ, HARM>;
+  const size_t lds = Lds<SH, EQ>::BYTES;
+  static int resident = 0;
+  if (lds > 65536)
+    WDY_HIP(hipFuncSetAttribute((const void *)fn,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  if (resident == 0) {
+    int dev = 0, cus = 0, per = 0;
+    WDY_HIP(hipGetDevice(&dev));
+    WDY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    WDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)fn,
+                                                         SH::WT, lds));
+    resident = (per > 0 ? per : 1) * (cus > 0 ? cus : 1);
+  }
+  const u32 grid = a.nb < (u32)resident ? a.nb : (u32)resident;
+  fn<<<grid, SH::WT, lds, st>>>(a);
+  WDY_CHECK_LAUNCH();
+  return WDY_OK;
+}
+
 template <class SH>
 int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
                  int mode) {
+  if (mode == 2) {  // mode 0, persistent pipelined kernel
+    if (eq)
+      return harm ? launch_pipe<SH, true, true>(st, a)
+                  : launch_pipe<SH, true, false>(st, a);
+    return harm ? launch_pipe<SH, false, true>(st, a)
+                : launch_pipe<SH, false, false>(st, a);
+  }
   if (mode == 0) {
     if (eq)
       return harm ? launch_one<SH, true, true, 0>(st, a)
@@ -518,7 +924,8 @@ int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
 // workgroups per CU, so one workgroup's memory phases overlap another's sort.
 static const WarmShape kShapes[WARM_NSHAPES] = {
     {1024, 4096, 4096}, {512, 2048, 2048}, {256, 1024, 1024}, {512, 1024, 1024},
-    {512, 2048, 1024}, {512, 3072, 1024}, {768, 3072, 768}};
+    {512, 2048, 1024}, {512, 3072, 1024}, {768, 3072, 768},
+    {256, 2048, 1024}, {256, 1536, 768}};
 
 const WarmShape &warm_shape(int v) { return kShapes[v]; }
 
@@ -531,7 +938,9 @@ int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
     case 3: return launch_shape<Shape<512, 1024>>(st, a, equal_mass, harm, mode);
     case 4: return launch_shape<Shape<512, 2048, 1024>>(st, a, equal_mass, harm, mode);
     case 5: return launch_shape<Shape<512, 3072, 1024>>(st, a, equal_mass, harm, mode);
-    default: return launch_shape<Shape<768, 3072, 768>>(st, a, equal_mass, harm, mode);
+    case 6: return launch_shape<Shape<768, 3072, 768>>(st, a, equal_mass, harm, mode);
+    case 7: return launch_shape<Shape<256, 2048, 1024>>(st, a, equal_mass, harm, mode);
+    default: return launch_shape<Shape<256, 1536, 768>>(st, a, equal_mass, harm, mode);
   }
 }
 
@@ -550,6 +959,9 @@ namespace wdy {
 
 struct WarmCtx {
   u32 T = 0, W = 0, S = 0, nb = 0, cap = 0;
+  bool pipe = true;  // mode 0 by the persistent pipelined kernel
+  bool fitted = false;  // window sized from the displacement estimate
+  bool tuned = false;   // set by wdy_tune: no automatic sizing
   int shape = WARM_SHAPE_DEFAULT;
   u32 *ring = nullptr;   // [8] ticket ring, overflow-count ring
   size_t seg_alloc = 0, nb_alloc = 0;  // allocated entries
@@ -635,6 +1047,9 @@ int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
     (void)frexp(s->totmass > 0.0 ? s->totmass : 1.0, &e);
     s->warm->fscale = ldexp(1.0, 62 - e);
     s->warm->finv = ldexp(1.0, e - 62);
+    // WDY_WARM_PIPE=0 selects the one-workgroup-per-bucket kernel instead
+    const char *pp = getenv("WDY_WARM_PIPE");
+    if (pp && pp[0] == '0') s->warm->pipe = false;
   }
   WarmCtx &w = *s->warm;
   const u32 n = (u32)s->n;
@@ -773,6 +1188,35 @@ int warm_partition_after_cold(wdy_sim *s, double hp) {
   return WDY_OK;
 }
 
+void warm_mark_tuned(wdy_sim *s) {
+  if (s->warm) s->warm->tuned = true;
+}
+
+int warm_fit(wdy_sim *s, double h) {
+  WarmCtx &w = *s->warm;
+  if (w.fitted || w.tuned) return WDY_OK;
+  w.fitted = true;
+  const double need = s->disp_rate * std::fabs(h);  // ranks per step, fast tail
+  if (!(need > 0.0) || !std::isfinite(need)) return WDY_OK;
+  // the current (default) shape, then larger buckets: the first whose window
+  // stays narrow (<= 12 buckets); else the largest buckets with the window
+  // they need
+  const int order[3] = {w.shape, WARM_SHAPE_EQ, 0};
+  for (int k = 0; k < 3; ++k) {
+    const int sh = order[k];
+    if (k > 0 && warm_shape(sh).cap <= warm_shape(order[k - 1]).cap) continue;
+    const u32 T = (u32)warm_shape(sh).cap / 4 * 3;
+    const u32 W = (u32)std::ceil(need / (double)T) + 1u;
+    if (W <= 12u || sh == 0) {
+      const u32 wmax = (u32)warm_shape(sh).wmax();
+      const u32 Wf = W < w.W ? w.W : (W < wmax ? W : wmax);
+      if (sh == w.shape && Wf == w.W) return WDY_OK;
+      return warm_configure(s, sh, 0u, Wf);
+    }
+  }
+  return WDY_OK;
+}
+
 int warm_current_shape(wdy_sim *s) {
   return s->warm ? s->warm->shape : WARM_SHAPE_DEFAULT;
 }
@@ -814,8 +1258,8 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     t = t + h;
     a.prof = w.prof;
     if (s->timing) timer_begin(s->timer, s->stream, KC_WARM);
-    int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0, 0,
-                              w.shape);
+    int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0,
+                              w.pipe && !w.prof ? 2 : 0, w.shape);
     if (s->timing) timer_end(s->timer, s->stream, KC_WARM);
     if (rc) return rc;
     in = out;
@@ -835,14 +1279,26 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
       return WDY_OK;
     }
     u32 T = w.T, W = w.W;
+    int shape = w.shape;
     if (kind & WARM_ERR_CAPACITY) T = T * 3 / 4 < WARM_T_MIN ? WARM_T_MIN : T * 3 / 4;
-    if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF))
-      W = W * 2 + 1 > (u32)WARM_WMAX ? (u32)WARM_WMAX : W * 2 + 1;
-    if (T == w.T && W == w.W) {  // limits reached: radix path from now on
-      warm_free(s);
+    if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) {
+      // sheets cross more buckets per step than the window holds: widen it,
+      // and once it is at its limit move to the largest buckets
+      const u32 wmax = (u32)warm_shape(shape).wmax();
+      if (W * 2 + 1 <= wmax) {
+        W = W * 2 + 1;
+      } else if (warm_shape(shape).cap < warm_shape(0).cap) {
+        shape = 0;
+        T = 0;
+      } else {
+        W = wmax;
+      }
+    }
+    if (shape == w.shape && T == w.T && W == w.W) {  // limits reached: radix
+      warm_free(s);                                  // path from now on
       return WDY_OK;
     }
-    if (int rc = warm_configure(s, -1, T, W)) return rc;
+    if (int rc = warm_configure(s, shape, T, W)) return rc;
     return WDY_OK;
   }
   s->cur = in;

@@ -4,8 +4,9 @@ advanced by continuous batching on one GPU.
 No reference counterpart (the reference runs one simulation per ``nbody``
 generator).  Each request is the approx=True leapfrog of
 ``wendy_amd.nbody(x, v, m, dt=tick*h, nleap=tick, omega=...)`` run for
-``steps`` leapfrog steps: every scheduling tick advances a simulation by one
-output interval of min(tick, steps left) steps of size h.
+``steps`` leapfrog steps: every output interval advances a simulation by
+min(tick, steps left) steps of size h (``tick`` for pool-resident requests,
+``tick_large`` for large-path ones).
 
 * Simulations with n <= ``split`` live in a ``Pool`` (wdy_pool_*): one device
   arena, one launch per tick for all of them, one workgroup per simulation.
@@ -189,13 +190,16 @@ class Server:
     ``budget_bytes``: HBM the served state may use (the pool arena counts in
     full); ``pool_sheets``/``max_sims``: the pool's arena and slot count;
     ``split``: largest n served from the pool; ``max_big``: large-path
-    simulations run at once; ``tick``: leapfrog steps per scheduling tick.
+    simulations run at once; ``tick``: leapfrog steps per scheduling tick
+    (one output interval) of the pool; ``tick_large``: steps per output
+    interval of a large-path simulation (default ``tick``).
     ``pool_factory``/``big_factory`` exist so the scheduling logic can be
     exercised without a GPU.
     """
 
     def __init__(self, budget_bytes=200e9, pool_sheets=None, max_sims=2048,
-                 split=1 << 16, max_big=8, tick=10, pool_factory=None,
+                 split=1 << 16, max_big=8, tick=10, tick_large=None,
+                 pool_factory=None,
                  big_factory=None, clock=time.perf_counter, sleep=time.sleep):
         if pool_sheets is None:
             pool_sheets = max_sims * split
@@ -206,6 +210,7 @@ class Server:
         self.split = int(split)
         self.max_big = int(max_big)
         self.tick = int(tick)
+        self.tick_large = int(tick if tick_large is None else tick_large)
         self.clock = clock
         self.sleep = sleep
         make_pool = pool_factory or (lambda: Pool(pool_sheets, max_sims, split))
@@ -277,7 +282,7 @@ class Server:
                 res = result_for(r, 'sim')
                 x, v, m = r.make()
                 b = self.big_factory(r, x, v, m)
-                big[workers.submit(_run_big, b, r, self.tick)] = res
+                big[workers.submit(_run_big, b, r, self.tick_large)] = res
                 big_bytes += need
                 qb.popleft()
 
