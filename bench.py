@@ -402,7 +402,7 @@ def main():
     achieved = bpl / avg_s if avg_s > 0 else 0.
     traffic, traffic_src = None, None
     if dom == KC_WARM and args.workload == 'plain24':
-        traffic, traffic_src = measured_traffic('warm_step_n16777216_hbm_traffic.md')
+        traffic, traffic_src = measured_traffic('warm_pipe_n16777216_hbm_traffic.md')
     roofline = {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9,
                 'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': traffic,
                 'traffic_source': traffic_src,
