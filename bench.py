@@ -24,9 +24,12 @@ sample of the same disks) and ``serving`` (a bounded window of the Poisson
 request load through wendy_amd.serve, p50/p99 completion latency).
 
 Roofline: the dominant kernel is timed with HIP events on the simulation's
-stream during the timed region (per-launch average); ``achieved`` is its
-algorithmic bytes per launch divided by that average, against the MI355X HBM
-peak of 8 TB/s.
+stream (an event pair around every launch) over a second timed region of the
+same work, min(K, 20) output intervals right after the first; ``achieved``
+is its algorithmic bytes per launch divided by that per-launch average,
+against the MI355X HBM peak of 8 TB/s.  ``traffic`` is the HBM bytes per
+launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE counters (committed
+summary under profiles/).
 
 CPU baseline: the reference's own C (built from its sources into oracle/_ref
 by oracle/Makefile, sort='parallel', OpenMP) timed on this host on a bounded
@@ -354,19 +357,31 @@ def main():
         sim.advance(h, nleap)
     sim.read()
     log('bench: warmup done')
-    sim.set_timing(True)
     sim.read()  # synchronise
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.advance(h, nleap)
     sim.read()  # synchronise
     t1 = time.perf_counter()
-    sim.set_timing(False)
     elapsed = t1 - t0
     rate = n * nleap * args.steps / elapsed
     log('bench: device-only %.3e sheet-steps/s (%.2f ms/output)'
         % (rate, 1e3 * elapsed / args.steps))
+    # per-kernel device time: a second timed region of the same work with a
+    # HIP event pair around every launch on the simulation's stream (the
+    # events themselves add a little time between launches, so the headline
+    # rate above is taken without them)
+    kb = max(1, min(args.steps, 20))
+    sim.set_timing(True)
+    for _ in range(kb):
+        sim.advance(h, nleap)
+    sim.read()
+    sim.set_timing(False)
     kt = {c: sim.kernel_time(c) for c in range(6)}
+    paths = {'warm_steps': sim.path_stats()[0], 'cold_steps': sim.path_stats()[1],
+             'redos': sim.path_stats()[2],
+             'failures_capacity_overflow_stall': list(sim.warm_failures())}
+    log('bench: paths %s' % paths)
 
     # end-to-end through the generator loop: advance + de-sort + D2H copy
     xo = numpy.empty(n)
@@ -436,6 +451,7 @@ def main():
                    'omega': omega},
         'e2e_value': e2e,
         'kernel_ms': {str(c): kt[c][0] for c in kt},
+        'paths': paths,
         'roofline': roofline,
         'cpu_baseline': cpu,
         'ensemble': ens,

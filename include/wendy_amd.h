@@ -101,12 +101,12 @@ int wdy_destroy(wdy_sim *sim);
  *   shape          tile shape of the warm kernel (-1 keeps the current one):
  *                  0: 1024 threads, buckets of up to 4096 sheets
  *                  1: 512 threads, up to 2048;  2: 256 threads, up to 1024
- *                  (default for unequal masses);  3: 512 threads, up to
+ *                  (default);  3: 512 threads, up to
  *                  1024;  4: 512 threads, up to 2048 with 1024 position
  *                  bins;  5: 512 threads, up to 3072 with 1024 bins;
  *                  6: 768 threads, up to 3072 with 768 bins;  7: 256
  *                  threads, up to 2048 with 1024 bins;  8: 256 threads, up
- *                  to 1536 with 768 bins (default for equal masses)
+ *                  to 1536 with 768 bins
  *   bucket_target  sheets per value bucket, WDY_BUCKET_MIN .. the shape's
  *                  capacity; 0 picks 3/4 of the capacity
  *   window         neighbouring buckets a sheet may move into per step
@@ -146,6 +146,13 @@ int wdy_time(wdy_sim *sim, double *t_out);
  * warm-start failure.  No reference counterpart. */
 int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
                    int64_t *redos);
+
+/* Warm intervals redone so far, by cause: a bucket outgrew its capacity,
+ * far movers overflowed their list, or a bounded wait of the look-back gave
+ * up (the last disables the warm path for the sim).  No reference
+ * counterpart. */
+int wdy_warm_failures(wdy_sim *sim, int64_t *capacity, int64_t *overflow,
+                      int64_t *stall);
 
 /* Per-kernel-class device time, for benchmarks.  wdy_set_timing(sim, 1)
  * brackets every subsequent launch with HIP events on the sim's stream;

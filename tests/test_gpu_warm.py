@@ -196,3 +196,19 @@ def test_warm_kernel_forms_agree(gpu_lib, monkeypatch, mass, pipe):
     x, v, m = make(9000, 22, mass)
     warm, _, _ = run_both(x, v, m, 0.02, 4, 3, tune=(64, 2))
     assert warm > 0
+
+
+def test_first_warm_interval_after_fitting_succeeds(gpu_lib):
+    """The window fitted to the sampled displacement (here wider than the
+    default) is applied before the first partition, so the first warm
+    interval does not fail."""
+    x, v, m = make(1 << 20, 30)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path='large')
+    try:
+        for _ in range(3):
+            sim.advance(0.01, 5)   # ~1e4 ranks per step for the fastest sheets
+        sim.read()
+        assert sim.path_stats() == (10, 5, 0)
+        assert sim.warm_failures() == (0, 0, 0)
+    finally:
+        sim.close()

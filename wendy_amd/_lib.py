@@ -35,7 +35,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
-            'wdy_set_ext_force', 'wdy_time',
+            'wdy_set_ext_force', 'wdy_time', 'wdy_warm_failures',
             'wdy_exact_create', 'wdy_exact_advance', 'wdy_exact_reset',
             'wdy_exact_read', 'wdy_exact_destroy',
             'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
@@ -123,6 +123,8 @@ def _declare(lib):
     lib.wdy_exact_read.argtypes = [S, S, S]
     lib.wdy_exact_destroy.restype = ctypes.c_int
     lib.wdy_exact_destroy.argtypes = [S]
+    lib.wdy_warm_failures.restype = ctypes.c_int
+    lib.wdy_warm_failures.argtypes = [S] + [ctypes.POINTER(ctypes.c_int64)] * 3
     lib.wdy_set_ext_force.restype = ctypes.c_int
     lib.wdy_set_ext_force.argtypes = [S, ctypes.c_int, _F64, ctypes.c_double]
     lib.wdy_time.restype = ctypes.c_int

@@ -262,7 +262,7 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     bool warm = !s->small && !(flags & WDY_NO_WARM);
     for (int64_t i = 0; warm && i < n; ++i) warm = m_id[i] >= 0.0;
     if (warm) {
-      r = warm_configure(s, eq ? WARM_SHAPE_EQ : WARM_SHAPE_UNEQ, 0u, 8u);
+      r = warm_configure(s, WARM_SHAPE_DEFAULT, 0u, 8u);
       if (r) return r;
       s->disp_rate = displacement_rate(n, x_slot, v_id, id_slot);
     }
@@ -301,6 +301,11 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
       }
       s->redos += 1;
     }
+    // size the warm-start layout for this h before the splitters of the
+    // next partition are written (a resize reallocates their buffers)
+    if (s->warm) {
+      if (int rc = warm_fit(s, h)) return rc;
+    }
     int rc = launch_drift(s, h / 2.);
     if (rc) return rc;
     u32 T = 0, nb = 0;
@@ -316,8 +321,6 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
     if (s->ext.kind) s->t0 = t;
     s->cold_steps += nleap;
     if (s->warm) {
-      rc = warm_fit(s, h);
-      if (rc) return rc;
       rc = warm_partition_after_cold(s, h / 2.);
       if (rc) return rc;
     }
@@ -462,6 +465,15 @@ int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
   if (warm_steps) *warm_steps = sim->warm_steps;
   if (cold_steps) *cold_steps = sim->cold_steps;
   if (redos) *redos = sim->redos;
+  return WDY_OK;
+}
+
+int wdy_warm_failures(wdy_sim *sim, int64_t *capacity, int64_t *overflow,
+                      int64_t *stall) {
+  if (int rc = check_sim(sim, "wdy_warm_failures")) return rc;
+  if (capacity) *capacity = sim->fail_capacity;
+  if (overflow) *overflow = sim->fail_overflow;
+  if (stall) *stall = sim->fail_stall;
   return WDY_OK;
 }
 

@@ -183,6 +183,9 @@ struct wdy_sim {
   double disp_rate = 0.0;
 
   int64_t warm_steps = 0, cold_steps = 0, redos = 0;
+  // warm intervals that failed, by kind: bucket over capacity, overflow list
+  // full (or too many far movers into one bucket), bounded wait exceeded
+  int64_t fail_capacity = 0, fail_overflow = 0, fail_stall = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   wdy::Timer *timer = nullptr;
   bool timing = false;

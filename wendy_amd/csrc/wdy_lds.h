@@ -189,8 +189,14 @@ __device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
       const u32 lo = q[i] == 0 ? 0u : s.bins[q[i] - 1], hi = s.bins[q[i]];
       const u64 k = s.key[e];
       const i32 id = s.id[e];
-      u32 r = lo;
-      for (u32 a = lo; a < hi; ++a) {
+      u32 r = lo, a = lo;
+      for (; a + 2 <= hi; a += 2) {  // two at a time: the LDS loads overlap
+        const unsigned short f0 = s.ord[a], f1 = s.ord[a + 1];
+        const u64 k0 = s.key[f0], k1 = s.key[f1];
+        const i32 i0 = s.id[f0], i1 = s.id[f1];
+        r += (pair_lt(k0, i0, k, id) ? 1u : 0u) + (pair_lt(k1, i1, k, id) ? 1u : 0u);
+      }
+      if (a < hi) {
         const unsigned short f = s.ord[a];
         r += pair_lt(s.key[f], s.id[f], k, id) ? 1u : 0u;
       }

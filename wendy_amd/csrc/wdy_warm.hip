@@ -132,6 +132,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   const u32 S = A.S, W = A.W, nb = A.nb;
+  const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
   // ---- 1. ticket, error word and overflow count: one round trip ------------
   if (t == 0) {
     s_u[0] = atomicAdd(A.tick, 1u);
@@ -424,10 +425,9 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
           }
         }
       }
-      // wave-level ranking on d (8 ballots cover d <= 255)
+      // wave-level ranking on d: one ballot per bit of d <= S
       u64 peers = __ballot(valid);
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < dbits; ++b) {
         const bool bit = (d >> b) & 1u;
         const u64 bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -500,6 +500,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   const u32 S = A.S, W = A.W, nb = A.nb;
+  const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
   if (t == 0) {
     s_u[0] = atomicAdd(A.tick, 1u);
     s_u[1] = ld_u32(A.err);
@@ -586,6 +587,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
   while (j < nb) {
     const u32 lo = j >= W ? j - W : 0u, hi = j + W < nb ? j + W : nb - 1;
     // ---- commit bucket j to LDS; publish its aggregate ----------------------
+    WARM_STAMP(0);
     if (t == 0) s_u[0] = atomicAdd(A.tick, 1u);  // the next ticket
     u64 msum = 0;
     if (ok) {
@@ -620,14 +622,17 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
                 c - nseg > (u32)LOVF ? WARM_ERR_LOVF : WARM_ERR_CAPACITY);
     }
     __syncthreads();
+    WARM_STAMP(1);
     const u32 jn = s_u[0];
     const int bn = b ^ 1;
     if (t == 0) s_novf[bn] = 0;
     if (jn < nb) table_issue(jn);  // in flight during the sort
 
     // ---- sort, look-back ---------------------------------------------------------
-    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm, nullptr);
+    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm,
+                                    A.prof ? A.prof + 8ull * j : nullptr);
     else __syncthreads();
+    WARM_STAMP(2);
     if (w == 0) {
       u64 ecnt = 0, emass = 0;
       bool stalled = false;
@@ -695,6 +700,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
       }
     }
     __syncthreads();
+    WARM_STAMP(3);
     const u32 off = (u32)s_u64[0];
     const u64 bfix = s_u64[1];
     const bool write = ok && !s_u[2];
@@ -752,6 +758,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
     }
     // ---- the next bucket's loads, in flight during the partition --------------------
     if (ok_n) data_issue(nseg_n, c_n, bn);
+    WARM_STAMP(4);
     if (write) {
       // ---- partition by next-step bucket and write out ---------------------------
       u32 *wc = s.bins;  // [NWAVE][S + 1] per-wave segment counters
@@ -793,8 +800,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
           }
         }
         u64 peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
+        for (int bit = 0; bit < dbits; ++bit) {
           const bool on = (d >> bit) & 1u;
           const u64 bal = __ballot(on);
           peers &= on ? bal : ~bal;
@@ -838,6 +844,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
       }
     }
     __syncthreads();  // LDS of bucket j free before the next commit
+    WARM_STAMP(5);
     j = jn;
     b = bn;
     nseg = nseg_n;
@@ -1012,11 +1019,13 @@ void warm_free(wdy_sim *s) {
       const double k = 1.0 / (double)w->prof_n;
       fprintf(stderr,
               "wendy_amd warm-step profile (n=%lld, %lld launches, mean us "
-              "per bucket): gather %.2f sort %.2f look-back %.2f force %.2f "
+              "per bucket): %s %.2f sort %.2f look-back %.2f %s %.2f "
               "partition+write %.2f | launch span %.1f | sort = histogram "
-              "%.2f + scan/scatter %.2f + fix-ups %.2f\n",
-              (long long)s->n, (long long)w->prof_n, w->prof_us[0] * k,
-              w->prof_us[1] * k, w->prof_us[2] * k, w->prof_us[3] * k,
+              "%.2f + scan/scatter %.2f + ranks %.2f\n",
+              (long long)s->n, (long long)w->prof_n,
+              w->pipe ? "commit" : "gather", w->prof_us[0] * k,
+              w->prof_us[1] * k, w->prof_us[2] * k,
+              w->pipe ? "next-tables+force" : "force", w->prof_us[3] * k,
               w->prof_us[4] * k, w->prof_us[5] * k, w->prof_us[6] * k,
               w->prof_us[7] * k, w->prof_us[8] * k);
     }
@@ -1201,7 +1210,7 @@ int warm_fit(wdy_sim *s, double h) {
   // the current (default) shape, then larger buckets: the first whose window
   // stays narrow (<= 12 buckets); else the largest buckets with the window
   // they need
-  const int order[3] = {w.shape, WARM_SHAPE_EQ, 0};
+  const int order[3] = {w.shape, WARM_SHAPE_MID, 0};
   for (int k = 0; k < 3; ++k) {
     const int sh = order[k];
     if (k > 0 && warm_shape(sh).cap <= warm_shape(order[k - 1]).cap) continue;
@@ -1259,7 +1268,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     a.prof = w.prof;
     if (s->timing) timer_begin(s->timer, s->stream, KC_WARM);
     int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0,
-                              w.pipe && !w.prof ? 2 : 0, w.shape);
+                              w.pipe ? 2 : 0, w.shape);
     if (s->timing) timer_end(s->timer, s->stream, KC_WARM);
     if (rc) return rc;
     in = out;
@@ -1271,6 +1280,9 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
   if (w.h_err[0] != 0) {
     const u32 kind = w.h_err[1];
     *failed = true;
+    if (kind & WARM_ERR_CAPACITY) s->fail_capacity += 1;
+    if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) s->fail_overflow += 1;
+    if (kind & WARM_ERR_STALL) s->fail_stall += 1;
     s->cur = start;
     w.rd = mstart;
     w.ready = false;

@@ -95,6 +95,12 @@ class Sim:
                                              ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def warm_failures(self):
+        """(capacity, overflow, stall): warm intervals redone, by cause."""
+        c = [ctypes.c_int64(0) for _ in range(3)]
+        _lib.check(_lib.lib().wdy_warm_failures(self._h, *[ctypes.byref(x) for x in c]))
+        return tuple(x.value for x in c)
+
     def set_timing(self, on=True):
         _lib.check(_lib.lib().wdy_set_timing(self._h, int(bool(on))))
 
