@@ -254,3 +254,33 @@ def test_approx_against_exact(gpu_lib, omega):
         txe, tve = next(ge)
         assert numpy.all(numpy.fabs(tx - txe) < 1e-5)
         assert numpy.all(numpy.fabs(tv - tve) < 1e-5)
+
+
+@pytest.mark.skipif(not R.available(), reason='oracle/_ref not built')
+@pytest.mark.parametrize('mass,omega', [('equal', 0.1), ('lognormal', None)])
+def test_trajectory_and_energy_drift_against_reference(gpu_lib, mass, omega):
+    """The acceptance bars against the compiled reference itself: after 100
+    leapfrog steps positions and velocities agree to 1e-10 relative, and the
+    relative energy drift is within 10% of the reference's own drift (plus a
+    1e-14 floor)."""
+    n = 1 << 14
+    x, v, m = O.sech2_disk(n, seed=17, mass='equal' if mass == 'equal'
+                           else ('lognormal', 0.5))
+    dt, nleap = 0.05, 10
+    ref = R.RefApprox(x, v, m, dt, nleap, omega=omega, sort='quick')
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.,
+                        path='large')
+    xo = numpy.empty(n)
+    vo = numpy.empty(n)
+    E0 = O.energy(x, v, m, omega=omega)
+    try:
+        for _ in range(10):
+            rx, rv, _ = ref.onestep()
+            sim.advance(dt / nleap, nleap)
+        sim.read(xo, vo)
+    finally:
+        sim.close()
+    assert rel_inf(xo, rx) <= 1e-10 and rel_inf(vo, rv) <= 1e-10
+    d_ref = abs((O.energy(rx, rv, m, omega=omega) - E0) / E0)
+    d_gpu = abs((O.energy(xo, vo, m, omega=omega) - E0) / E0)
+    assert abs(d_gpu - d_ref) <= 0.1 * d_ref + 1e-14, (d_gpu, d_ref)

@@ -42,6 +42,7 @@
 // exceeds its bounded wait) records the failing launch's epoch in err[0];
 // later launches exit at once, and the host redoes the interval from its
 // untouched start state on the radix path.
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -881,20 +882,23 @@ template <class SH, bool EQ, bool HARM>
 int launch_pipe(hipStream_t st, const WarmArgs &a) {
   auto fn = k_warm_pipe<SH, EQ, HARM>;
   const size_t lds = Lds<SH, EQ>::BYTES;
-  static int resident = 0;
+  static std::atomic<int> resident[64];  // per device; 0: not computed yet
   if (lds > 65536)
     WDY_HIP(hipFuncSetAttribute((const void *)fn,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  if (resident == 0) {
-    int dev = 0, cus = 0, per = 0;
-    WDY_HIP(hipGetDevice(&dev));
+  int dev = 0;
+  WDY_HIP(hipGetDevice(&dev));
+  int res = dev < 64 ? resident[dev].load() : 0;
+  if (res == 0) {
+    int cus = 0, per = 0;
     WDY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     WDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)fn,
                                                          SH::WT, lds));
-    resident = (per > 0 ? per : 1) * (cus > 0 ? cus : 1);
+    res = (per > 0 ? per : 1) * (cus > 0 ? cus : 1);
+    if (dev < 64) resident[dev].store(res);
   }
-  const u32 grid = a.nb < (u32)resident ? a.nb : (u32)resident;
+  const u32 grid = a.nb < (u32)res ? a.nb : (u32)res;
   fn<<<grid, SH::WT, lds, st>>>(a);
   WDY_CHECK_LAUNCH();
   return WDY_OK;
