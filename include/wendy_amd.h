@@ -87,6 +87,22 @@ int wdy_advance(wdy_sim *sim, double h, int nleap);
  * Either array pointer may be NULL. */
 int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out);
 
+/* One generator step in a single call: de-sort the current state, start the
+ * next output interval (h, nleap) on the device, and copy the de-sorted x, v
+ * to the host while that interval runs; returns once the copy is complete
+ * (the interval may still be running).  seconds_out: device time of the
+ * interval just read.  With host buffers from wdy_host_alloc the copy
+ * overlaps the interval; with pageable memory the result is the same but the
+ * copy may not overlap.  The counterpart of one next() of the reference
+ * generator (wendy/wendy.py:424-437) with the following interval started
+ * ahead of the caller's request. */
+int wdy_read_and_advance(wdy_sim *sim, double *x_out, double *v_out, double h,
+                         int nleap, double *seconds_out);
+
+/* Page-locked host memory for output buffers (hipHostMalloc). */
+int wdy_host_alloc(int64_t bytes, void **out);
+int wdy_host_free(void *ptr);
+
 /* Total energy of the current device state, computed on the device without
  * copying the state out: twopiG * sum_{i<j} m_i m_j |x_i - x_j| +
  * sum m v^2/2 (+ sum m omega^2 x^2/2 when omega2 >= 0), with m the masses

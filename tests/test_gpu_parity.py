@@ -158,3 +158,40 @@ def test_large_n_bitwise_against_oracle(gpu_lib):
     got = run_sim(x, v, m, 0.005, 3, 1, omega=0.1)
     st = O.ApproxState(x, v, m, 0.005, 3, omega=0.1)
     assert_parity(got[0], st.onestep(), True, 'n=2^21')
+
+
+@pytest.mark.parametrize('n,path', [(3000, None), (50000, 'large')])
+def test_read_and_advance_equals_step_by_step(gpu_lib, n, path):
+    """The overlapped generator step (de-sort, start the next interval, copy
+    out) yields exactly what advance-then-read yields, with page-locked or
+    pageable output arrays, and the yielded arrays outlive the generator."""
+    from wendy_amd import _lib
+    x, v, m = O.sech2_disk(n, seed=n + 1)
+    h, nleap = 0.001, 7
+    ref = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path=path)
+    a = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path=path)
+    rx, rv = numpy.empty(n), numpy.empty(n)
+    px, pv = _lib.pinned_empty(n), _lib.pinned_empty(n)
+    qx, qv = numpy.empty(n), numpy.empty(n)
+    try:
+        a.advance(h, nleap)
+        for k in range(4):
+            ref.advance(h, nleap)
+            ref.read(rx, rv)
+            if k % 2 == 0:
+                a.read_and_advance(px, pv, h, nleap)
+                gx, gv = px, pv
+            else:
+                a.read_and_advance(qx, qv, h, nleap)
+                gx, gv = qx, qv
+            assert bits_equal(gx, rx) and bits_equal(gv, rv), k
+    finally:
+        ref.close()
+        a.close()
+    g = wendy_amd.nbody(x, v, m, h * nleap, approx=True, nleap=nleap, omega=0.1)
+    out = [next(g) for _ in range(3)]
+    assert out[0][0] is out[2][0]          # the generator's own arrays
+    keep = out[2][0]
+    g.close()
+    del g, out
+    assert numpy.isfinite(keep).all()     # still valid after the generator

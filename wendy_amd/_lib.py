@@ -36,6 +36,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
             'wdy_set_ext_force', 'wdy_time', 'wdy_warm_failures',
+            'wdy_read_and_advance', 'wdy_host_alloc', 'wdy_host_free',
             'wdy_exact_create', 'wdy_exact_advance', 'wdy_exact_reset',
             'wdy_exact_read', 'wdy_exact_destroy',
             'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
@@ -123,6 +124,13 @@ def _declare(lib):
     lib.wdy_exact_read.argtypes = [S, S, S]
     lib.wdy_exact_destroy.restype = ctypes.c_int
     lib.wdy_exact_destroy.argtypes = [S]
+    lib.wdy_read_and_advance.restype = ctypes.c_int
+    lib.wdy_read_and_advance.argtypes = [S, S, S, ctypes.c_double, ctypes.c_int,
+                                         _c_double_p]
+    lib.wdy_host_alloc.restype = ctypes.c_int
+    lib.wdy_host_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(S)]
+    lib.wdy_host_free.restype = ctypes.c_int
+    lib.wdy_host_free.argtypes = [S]
     lib.wdy_warm_failures.restype = ctypes.c_int
     lib.wdy_warm_failures.argtypes = [S] + [ctypes.POINTER(ctypes.c_int64)] * 3
     lib.wdy_set_ext_force.restype = ctypes.c_int
@@ -185,3 +193,31 @@ def check(rc):
 
 def f64(a):
     return numpy.ascontiguousarray(a, dtype=numpy.float64)
+
+
+class _HostBlock:
+    """Page-locked host memory from wdy_host_alloc, freed when the last array
+    viewing it is gone."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().wdy_host_alloc(int(nbytes), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().wdy_host_free(self.ptr)
+        except Exception:
+            pass
+        self.ptr = None
+
+
+def pinned_empty(n):
+    """A float64[n] numpy array in page-locked host memory (device-to-host
+    copies into it run at full speed and asynchronously)."""
+    n = int(n)
+    blk = _HostBlock(8 * max(n, 1))
+    carr = (ctypes.c_double * max(n, 1)).from_address(blk.ptr)
+    carr._owner = blk
+    return numpy.ctypeslib.as_array(carr)[:n]

@@ -134,6 +134,9 @@ static void free_sim(wdy_sim *s) {
   radix_free(&s->radix);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+  if (s->ev_copied) (void)hipEventDestroy(s->ev_copied);
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   if (s->timer) timer_destroy(s->timer);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -210,8 +213,11 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     s->small = (flags & WDY_FORCE_SMALL) ||
                (!(flags & WDY_FORCE_LARGE) && (u64)n <= smax);
     WDY_HIP(hipStreamCreate(&s->stream));
+    WDY_HIP(hipStreamCreate(&s->copy_stream));
     WDY_HIP(hipEventCreate(&s->ev0));
     WDY_HIP(hipEventCreate(&s->ev1));
+    WDY_HIP(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+    WDY_HIP(hipEventCreateWithFlags(&s->ev_copied, hipEventDisableTiming));
     s->timer = timer_create();
     if (!s->timer) return fail(WDY_ENOMEM, "wdy_create: timer allocation failed");
     s->n = n;
@@ -387,6 +393,54 @@ int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out) {
     if (hipEventElapsedTime(&ms, sim->ev0, sim->ev1) != hipSuccess) ms = 0.f;
     *seconds_out = ms * 1e-3;
   }
+  return WDY_OK;
+}
+
+int wdy_read_and_advance(wdy_sim *sim, double *x_out, double *v_out, double h,
+                         int nleap, double *seconds_out) {
+  if (int rc = check_sim(sim, "wdy_read_and_advance")) return rc;
+  if (nleap < 1) return fail(WDY_EINVAL, "wdy_read_and_advance: nleap=%d < 1", nleap);
+  if (!std::isfinite(h)) return fail(WDY_EINVAL, "wdy_read_and_advance: h not finite");
+  if (sim->small) {  // velocities live in the state buffer: read, then advance
+    if (int rc = wdy_read(sim, x_out, v_out, seconds_out)) return rc;
+    return wdy_advance(sim, h, nleap);
+  }
+  WDY_HIP(hipSetDevice(sim->device));
+  const size_t N = (size_t)sim->n;
+  WDY_HIP(hipEventSynchronize(sim->ev1));  // the interval being read
+  if (seconds_out) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, sim->ev0, sim->ev1) != hipSuccess) ms = 0.f;
+    *seconds_out = ms * 1e-3;
+  }
+  if (x_out || v_out) {
+    if (int rc = launch_desort(sim, sim->out_x, sim->out_v, nullptr, nullptr, nullptr))
+      return rc;
+    WDY_HIP(hipEventRecord(sim->ev_out, sim->stream));
+    WDY_HIP(hipStreamWaitEvent(sim->copy_stream, sim->ev_out, 0));
+    if (x_out)
+      WDY_HIP(hipMemcpyAsync(x_out, sim->out_x, N * 8, hipMemcpyDeviceToHost,
+                             sim->copy_stream));
+    if (v_out)
+      WDY_HIP(hipMemcpyAsync(v_out, sim->out_v, N * 8, hipMemcpyDeviceToHost,
+                             sim->copy_stream));
+    WDY_HIP(hipEventRecord(sim->ev_copied, sim->copy_stream));
+  }
+  // the next interval never touches the output staging buffers
+  const int rc = advance_impl(sim, h, nleap, nullptr, nullptr);
+  if (x_out || v_out) WDY_HIP(hipEventSynchronize(sim->ev_copied));
+  return rc;
+}
+
+int wdy_host_alloc(int64_t bytes, void **out) {
+  if (!out || bytes < 0) return fail(WDY_EINVAL, "wdy_host_alloc: bad arguments");
+  *out = nullptr;
+  WDY_HIP(hipHostMalloc(out, (size_t)(bytes > 0 ? bytes : 1), hipHostMallocDefault));
+  return WDY_OK;
+}
+
+int wdy_host_free(void *p) {
+  if (p) WDY_HIP(hipHostFree(p));
   return WDY_OK;
 }
 

@@ -148,6 +148,10 @@ struct wdy_sim {
   int64_t n = 0;
   int device = 0;
   hipStream_t stream = nullptr;
+  // output copies: device-to-host on their own stream so that they overlap
+  // the next interval (wdy_read_and_advance)
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_out = nullptr, ev_copied = nullptr;
   bool equal_mass = false;
   double m_equal = 0.0;      // mass * twopiG of every sheet (equal-mass runs)
   double totmass = 0.0;      // numpy.sum of mass * twopiG, supplied by the host

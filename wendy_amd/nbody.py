@@ -74,6 +74,23 @@ class Sim:
         _lib.check(_lib.lib().wdy_read(self._h, xp, vp, ctypes.byref(sec)))
         return sec.value
 
+    def read_and_advance(self, x_out, v_out, h, nleap):
+        """De-sort the current state into ``x_out``/``v_out``, starting the
+        next interval (h, nleap) on the device while the copy runs (see
+        wdy_read_and_advance); returns the device seconds of the interval
+        read.  Arrays from ``_lib.pinned_empty`` let the copy overlap."""
+        for a in (x_out, v_out):
+            if a is not None and (a.dtype != numpy.float64 or a.shape != (self.n,)
+                                  or not a.flags['C_CONTIGUOUS']
+                                  or not a.flags['WRITEABLE']):
+                raise ValueError('output arrays must be writeable float64[n]')
+        sec = ctypes.c_double(0.)
+        xp = None if x_out is None else x_out.ctypes.data_as(ctypes.c_void_p)
+        vp = None if v_out is None else v_out.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(_lib.lib().wdy_read_and_advance(self._h, xp, vp, float(h),
+                                                   int(nleap), ctypes.byref(sec)))
+        return sec.value
+
     def energy(self, twopiG=1., omega=None):
         """Total energy of the live device state (masses = m / twopiG)."""
         e = ctypes.c_double(0.)
@@ -152,9 +169,11 @@ def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
     them use the device sort, which orders exactly tied positions by original
     index.  Yields ``(x, v)`` or, with ``full_output``, ``(x, v, seconds)``
     where seconds is the device time of the interval.  The yielded arrays are
-    the generator's own and are overwritten by the next ``next()``, as in the
-    reference; unlike the reference, editing them does not feed back into the
-    simulation (its state lives on the GPU).
+    the generator's own (in page-locked host memory) and are overwritten by
+    the next ``next()``, as in the reference; unlike the reference, editing
+    them does not feed back into the simulation (its state lives on the GPU),
+    and the device computes the next interval while the caller works on the
+    current one.
 
     ``ext_force`` must be one of the device-evaluated forms of
     ``wendy_amd.forces`` (Harmonic, Isothermal, Pulse), which are also plain
@@ -255,14 +274,18 @@ def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
     if ext_force is not None:
         sim.set_ext_force(ext_force, t0)
     dt_leap = dt / nleap
+    # the yielded arrays live in page-locked host memory: each next() copies
+    # the finished interval out while the following one already runs
+    xo = _lib.pinned_empty(len(x))
+    vo = _lib.pinned_empty(len(x))
     try:
+        sim.advance(dt_leap, nleap)
         while True:
-            sim.advance(dt_leap, nleap)
-            seconds = sim.read(x, v)
+            seconds = sim.read_and_advance(xo, vo, dt_leap, nleap)
             if full_output:
-                yield (x, v, seconds)
+                yield (xo, vo, seconds)
             else:
-                yield (x, v)
+                yield (xo, vo)
     finally:
         sim.close()
 
