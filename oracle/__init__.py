@@ -12,9 +12,12 @@ Contents
                   (wendy/wendy.py:336-437, wendy/wendy.c:318-418), its energy
                   and momentum diagnostics (wendy/wendy.py:439-517), plus
                   closed forms used to pin it.
-``ref_harness``   ctypes driver for the reference C library built from the
+``ref_harness``   ctypes drivers for the reference C library built from the
                   reference's own sources by ``oracle/Makefile`` into
-                  ``oracle/_ref/`` (present only where it was built).
+                  ``oracle/_ref/`` (present only where it was built): the
+                  approx step (``RefApprox``, with an optional external force
+                  passed through a ctypes callback as wendy/wendy.py:406-418
+                  does) and the exact event-driven solver (``RefExact``).
 
 Pinning status of every oracle function (what checks it, so that a dropped
 term, a wrong sign or index, or a transposed operand fails a test):
@@ -25,6 +28,12 @@ term, a wrong sign or index, or a transposed operand fails a test):
   against closed forms: the two-sheet free-fall parabola (constant force,
   where drift-kick-drift is exact) and the harmonic rotation map of tracer
   sheets.
+* ``ext_force_sorted`` and the external-force clock of ``ApproxState`` --
+  bitwise against the compiled reference driven with the same callable, on
+  both sides of EXTERNAL_SWITCH, clock included; against the closed form of
+  massless sheets in an external harmonic force (equal to the built-in omega
+  term); and against the momentum change a uniform time-dependent force must
+  produce, M h sum_k g(t0 + k h), which a half-step clock error fails.
 * ``canonical_order`` -- against brute-force ranking of tiny inputs.
 * ``force_sorted`` -- against the brute-force pair sum
   ``sum_j m_j sign(x_j - x_i)`` with ``math.fsum``.
