@@ -212,3 +212,29 @@ def test_first_warm_interval_after_fitting_succeeds(gpu_lib):
         assert sim.warm_failures() == (0, 0, 0)
     finally:
         sim.close()
+
+
+def test_autotune_picks_a_shape_and_keeps_results(gpu_lib):
+    """Sim.autotune times candidate tile shapes on a copy of the state; the
+    simulation itself continues unchanged (its trajectory still matches the
+    oracle)."""
+    x, v, m = make(60000, 40)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path='large')
+    st = O.ApproxState(x, v, m, 0.005, 5, omega=0.1)
+    xo = numpy.empty(len(x))
+    vo = numpy.empty(len(x))
+    try:
+        sim.advance(0.001, 5)
+        times = sim.autotune(0.001, 5, candidates=((2, 8), (0, 8)), trials=1)
+        assert set(times) == {(2, 8), (0, 8)}
+        assert all(t > 0 for t in times.values())
+        sim.read(xo, vo)
+        wx, wv = st.onestep()
+        assert bits_equal(xo, wx) or rel_inf(xo, wx) <= TOL
+        for _ in range(2):
+            sim.advance(0.001, 5)
+            sim.read(xo, vo)
+            wx, wv = st.onestep()
+            assert rel_inf(xo, wx) <= TOL and rel_inf(vo, wv) <= TOL
+    finally:
+        sim.close()

@@ -56,6 +56,9 @@ class Sim:
         self._h = h
         self.n = len(x)
         self.equal_mass = equal_mass
+        self._m = m
+        self._totmass = float(totmass)
+        self._omega2 = float(omega2)
 
     def advance(self, h, nleap):
         _lib.check(_lib.lib().wdy_advance(self._h, float(h), int(nleap)))
@@ -111,6 +114,40 @@ class Sim:
         _lib.check(_lib.lib().wdy_path_stats(self._h, ctypes.byref(a),
                                              ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def autotune(self, h, nleap, candidates=((2, 8), (8, 8), (0, 8)), trials=2):
+        """Pick the warm-start tile shape by timing: each candidate (shape,
+        window) runs ``trials`` warm output intervals of (h, nleap) on a
+        temporary copy of the current state; the fastest is applied to this
+        simulation (from its next interval).  Returns {candidate: seconds per
+        interval}.  Large path only; the copy is made from the current
+        positions and velocities and the masses given at construction."""
+        import time
+        x = numpy.empty(self.n)
+        v = numpy.empty(self.n)
+        self.read(x, v)
+        m = numpy.array(self._m, copy=True)
+        times = {}
+        for shape, window in candidates:
+            tmp = Sim(x, v, m, self._totmass, self._omega2,
+                      equal_mass=self.equal_mass, path='large')
+            try:
+                tmp.tune(0, window, shape)
+                tmp.advance(h, nleap)          # re-layout (cold) interval
+                tmp.read()
+                t0 = time.perf_counter()
+                for _ in range(trials):
+                    tmp.advance(h, nleap)
+                tmp.read()
+                dt = (time.perf_counter() - t0) / trials
+                if tmp.path_stats()[2]:         # redone: not a fit
+                    dt = float('inf')
+                times[(shape, window)] = dt
+            finally:
+                tmp.close()
+        best = min(times, key=times.get)
+        self.tune(0, best[1], best[0])
+        return times
 
     def warm_failures(self):
         """(capacity, overflow, stall): warm intervals redone, by cause."""
@@ -276,8 +313,12 @@ def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
     dt_leap = dt / nleap
     # the yielded arrays live in page-locked host memory: each next() copies
     # the finished interval out while the following one already runs
-    xo = _lib.pinned_empty(len(x))
-    vo = _lib.pinned_empty(len(x))
+    try:
+        xo = _lib.pinned_empty(len(x))
+        vo = _lib.pinned_empty(len(x))
+    except (MemoryError, _lib.WendyError):  # no page-locked memory: pageable
+        xo = numpy.empty(len(x))
+        vo = numpy.empty(len(x))
     try:
         sim.advance(dt_leap, nleap)
         while True:
