@@ -780,6 +780,8 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
           const bool below = lo > 0 && kp < spl[0];
           const bool above = hi + 1 < nb && kp >= spl[hi - lo + 1];
           if (!below && !above) {
+            // the largest a with spl[a] <= kp (bisection: the same trip
+            // count for every lane of the wave)
             u32 a = 0, z = hi - lo;
             while (a < z) {
               const u32 mid = (a + z + 1) >> 1;
