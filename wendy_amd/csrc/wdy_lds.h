@@ -48,6 +48,29 @@ __device__ u32 block_excl_u32_inplace(u32 *s, u32 n, u64 *s_red) {
   return (u32)tot;
 }
 
+// The same for a short array: for n <= 64 one wave scans with shuffles and a
+// single barrier publishes the result (and every LDS write made before the
+// call).  The values must have been written by wave 0 (threads 0..63) or be
+// visible to it already.
+template <int NWAVE>
+__device__ u32 short_excl_u32_inplace(u32 *s, u32 n, u64 *s_red, u32 *s_tot) {
+  if (n > 64) return block_excl_u32_inplace<NWAVE>(s, n, s_red);
+  const u32 t = threadIdx.x;
+  if (t < 64) {
+    const u32 v = t < n ? s[t] : 0u;
+    u32 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const u32 y = __shfl_up(x, o, 64);
+      if (t >= (u32)o) x += y;
+    }
+    if (t < n) s[t] = x - v;
+    if (t == 63) *s_tot = x;
+  }
+  __syncthreads();
+  return *s_tot;
+}
+
 __device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
   return ka < kb || (ka == kb && ia < ib);
 }

@@ -494,6 +494,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
   __shared__ u32 s_segpre[2][SMAX + 1];
   __shared__ u32 s_oidx[2][LOVF];
   __shared__ u32 s_novf[2];
+  __shared__ u32 s_tot;
   __shared__ u64 s_red[NWAVE];
   __shared__ u64 s_mm[2 * NWAVE];
   __shared__ u32 s_u[4];
@@ -549,8 +550,8 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
         const u32 slot = atomicAdd(&s_novf[bb], 1u);
         if (slot < (u32)LOVF) s_oidx[bb][slot] = q2;
       }
-    __syncthreads();
-    return block_excl_u32_inplace<NWAVE>(s_segpre[bb], S, s_red);
+    if (S > 64) __syncthreads();  // the long scan reads other waves' counts
+    return short_excl_u32_inplace<NWAVE>(s_segpre[bb], S, s_red, &s_tot);
   };
 
   // data registers of the bucket being prefetched
@@ -825,8 +826,8 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
         }
         segp[t] = tot;
       }
-      __syncthreads();
-      block_excl_u32_inplace<NWAVE>(segp, S + 1, s_red);
+      if (S + 1 > 64) __syncthreads();
+      (void)short_excl_u32_inplace<NWAVE>(segp, S + 1, s_red, &s_tot);
       if (t <= S) {
         const u32 nxt = t < S ? segp[t + 1] : c;
         A.wr.segoff[j * (S + 1) + t] = segp[t];
