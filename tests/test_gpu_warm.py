@@ -238,3 +238,17 @@ def test_autotune_picks_a_shape_and_keeps_results(gpu_lib):
             assert rel_inf(xo, wx) <= TOL and rel_inf(vo, wv) <= TOL
     finally:
         sim.close()
+
+
+def test_warm_large_groups_of_exact_ties(gpu_lib):
+    """Massless sheets on a coarse grid of positions and velocities: groups
+    of hundreds of exactly tied sheets persist through every step (equal
+    (x, v), equal force), spanning bucket boundaries. The warm path sorts
+    them by (position, index) and matches the oracle bit for bit."""
+    rng = numpy.random.default_rng(5)
+    n = 40000
+    x = numpy.round(rng.standard_normal(n) * 8.) / 8.
+    v = numpy.round(rng.standard_normal(n) * 4.) / 4.
+    m = numpy.zeros(n)
+    warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, omega=0.7)
+    assert warm + cold == 20
