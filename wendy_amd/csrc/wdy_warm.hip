@@ -325,6 +325,8 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
     // ---- 6. force, kick, drift in sorted order -----------------------------------
     {
       const u32 r0 = t * PER;  // ranks r0 .. r0 + PER - 1, in order
+      // the one splitter rank (a multiple of T >= PER) among them, if any
+      const u32 sidx = (off + r0 + A.T - 1) / A.T, srank = sidx * A.T;
       u64 run = 0;
       if (!EQ) {
         u64 part = 0;
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
           x = x + A.hd * v;
           s.v[e] = v;
           s.key[e] = encode_pos(x);
-          if (R % A.T == 0) A.wr.spl[R / A.T] = R == 0 ? 0ull : kold;
+          if (R == srank) A.wr.spl[sidx] = R == 0 ? 0ull : kold;
         }
       }
       if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
@@ -721,6 +723,9 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
       // ---- force, kick, drift in sorted order ------------------------------------
       {
         const u32 r0 = t * PER;
+        // the one splitter rank (a multiple of T) this thread's PER <= T
+        // consecutive ranks may contain: one division per thread
+        const u32 sidx = (off + r0 + A.T - 1) / A.T, srank = sidx * A.T;
         u64 run = 0;
         if (!EQ) {
           u64 part = 0;
@@ -751,7 +756,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
             x = x + A.hd * v;
             s.v[e] = v;
             s.key[e] = encode_pos(x);
-            if (R % A.T == 0) A.wr.spl[R / A.T] = R == 0 ? 0ull : kold;
+            if (R == srank) A.wr.spl[sidx] = R == 0 ? 0ull : kold;
           }
         }
         if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
