@@ -154,3 +154,39 @@ def test_batch_create_rejects_bad_arguments_before_touching_device():
     assert lib.wdy_batch_create(0, ok, y, y, y, one, om,
                                 ctypes.byref(h)) == _lib.WDY_EINVAL
     assert h.value is None
+
+
+def test_exact_create_rejects_bad_arguments():
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    y = numpy.array([0., 1., 2.])
+    one = numpy.ones(3)
+    assert lib.wdy_exact_create(0, y, y, one, -1., ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert lib.wdy_exact_create(3, y, y, numpy.array([1., 0., 1.]), -1.,
+                                ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert lib.wdy_exact_create(3, numpy.array([0., numpy.nan, 1.]), y, one, -1.,
+                                ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert h.value is None
+    # a valid one works without any GPU (the exact solver is host code)
+    assert lib.wdy_exact_create(3, y, numpy.zeros(3), one, -1., ctypes.byref(h)) == 0
+    n = ctypes.c_int64(-1)
+    hit = ctypes.c_int(-1)
+    sec = ctypes.c_double(-1.)
+    assert lib.wdy_exact_advance(h, 0., 10, ctypes.byref(n), ctypes.byref(hit),
+                                 ctypes.byref(sec)) == _lib.WDY_EINVAL
+    assert lib.wdy_exact_advance(h, 0.1, 10, ctypes.byref(n), ctypes.byref(hit),
+                                 ctypes.byref(sec)) == 0
+    assert n.value == 0 and hit.value == 0
+    assert lib.wdy_exact_destroy(h) == 0
+
+
+def test_gpu_handles_report_no_device_here():
+    """Without a GPU the pool and the page-locked allocator fail cleanly."""
+    lib = _lib.lib()
+    n = ctypes.c_int(0)
+    if lib.wdy_device_count(ctypes.byref(n)) == 0 and n.value > 0:
+        pytest.skip('a GPU is visible')
+    h = ctypes.c_void_p()
+    assert lib.wdy_pool_create(1000, 4, 100, ctypes.byref(h)) == _lib.WDY_ENODEV
+    assert lib.wdy_pool_create(10, 4, 100, ctypes.byref(h)) == _lib.WDY_EINVAL
+    assert h.value is None
