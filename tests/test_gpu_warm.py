@@ -252,3 +252,30 @@ def test_warm_large_groups_of_exact_ties(gpu_lib):
     m = numpy.zeros(n)
     warm, cold, redo = run_both(x, v, m, 0.02, 5, 4, omega=0.7)
     assert warm + cold == 20
+
+
+def test_warm_cold_collapse(gpu_lib):
+    """A uniform slab released from rest collapses: velocities grow from zero
+    to order one within a few outputs, so the window sized at the start is
+    soon too narrow. Whatever the warm path does (widen, move to larger
+    buckets, redo on the radix path), the trajectory stays the oracle's."""
+    rng = numpy.random.default_rng(11)
+    n = 1 << 16
+    x = rng.uniform(-1., 1., n)
+    v = numpy.zeros(n)
+    m = numpy.full(n, 1. / n)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large')
+    st = O.ApproxState(x, v, m, 0.2, 10)
+    xo = numpy.empty(n)
+    vo = numpy.empty(n)
+    try:
+        for k in range(8):    # through the first collapse (t ~ 1.4)
+            sim.advance(0.02, 10)
+            sim.read(xo, vo)
+            wx, wv = st.onestep()
+            assert bits_equal(xo, wx) and bits_equal(vo, wv), k
+        warm, cold, redo = sim.path_stats()
+        print('collapse paths', warm, cold, redo, sim.warm_failures(), flush=True)
+        assert warm + cold == 80
+    finally:
+        sim.close()
