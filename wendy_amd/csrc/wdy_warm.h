@@ -19,11 +19,14 @@ struct WarmShape {
   }
 };
 constexpr int WARM_NSHAPES = 9;
-// default tile shape (measured at N = 2^24 with the pipelined kernel): 256
-// threads per workgroup, buckets of up to 1024 sheets, four workgroups per
-// CU.  When sheets cross many buckets per step the window fitting moves to
-// the 1536-sheet shape, then to the 4096-sheet one.
-constexpr int WARM_SHAPE_DEFAULT = 2;
+// default tile shapes (measured at N = 2^24 with the pipelined kernel), 256
+// threads per workgroup: buckets of up to 1536 sheets for equal masses
+// (three workgroups per CU), 1024 for unequal ones (four per CU).  When
+// sheets cross many buckets per step the window fitting moves to larger
+// buckets (1536, then 4096 sheets).
+constexpr int WARM_SHAPE_EQ = 8;
+constexpr int WARM_SHAPE_UNEQ = 2;
+constexpr int WARM_SHAPE_DEFAULT = WARM_SHAPE_UNEQ;
 constexpr int WARM_SHAPE_MID = 8;
 const WarmShape &warm_shape(int v);
 constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry

@@ -1076,7 +1076,7 @@ int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
   const u32 n = (u32)s->n;
   if (shape >= 0) w.shape = shape;
   w.cap = (u32)warm_shape(w.shape).cap;
-  if (T == 0) T = w.cap / 4 * 3;
+  if (T == 0) T = w.cap / 5 * 4;  // default fill: 4/5 of the capacity
   if (T > w.cap) T = w.cap;
   w.T = T;
   w.W = W;
@@ -1226,7 +1226,7 @@ int warm_fit(wdy_sim *s, double h) {
   for (int k = 0; k < 3; ++k) {
     const int sh = order[k];
     if (k > 0 && warm_shape(sh).cap <= warm_shape(order[k - 1]).cap) continue;
-    const u32 T = (u32)warm_shape(sh).cap / 4 * 3;
+    const u32 T = (u32)warm_shape(sh).cap / 5 * 4;
     const u32 W = (u32)std::ceil(need / (double)T) + 1u;
     if (W <= 12u || sh == 0) {
       const u32 wmax = (u32)warm_shape(sh).wmax();
@@ -1304,7 +1304,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     }
     u32 T = w.T, W = w.W;
     int shape = w.shape;
-    if (kind & WARM_ERR_CAPACITY) T = T * 3 / 4 < WARM_T_MIN ? WARM_T_MIN : T * 3 / 4;
+    if (kind & WARM_ERR_CAPACITY) T = T * 7 / 8 < WARM_T_MIN ? WARM_T_MIN : T * 7 / 8;
     if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) {
       // sheets cross more buckets per step than the window holds: widen it,
       // and once it is at its limit move to the largest buckets
