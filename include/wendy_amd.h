@@ -116,19 +116,20 @@ int wdy_destroy(wdy_sim *sim);
 /* Warm-start tuning of a large-path sim.
  *   shape          tile shape of the warm kernel (-1 keeps the current one):
  *                  0: 1024 threads, buckets of up to 4096 sheets
- *                  1: 512 threads, up to 2048;  2: 256 threads, up to 1024
- *                  (default for unequal masses);  3: 512 threads, up to
- *                  1024;  4: 512 threads, up to 2048 with 1024 position
- *                  bins;  5: 512 threads, up to 3072 with 1024 bins;
- *                  6: 768 threads, up to 3072 with 768 bins;  7: 256
- *                  threads, up to 2048 with 1024 bins;  8: 256 threads, up
- *                  to 1536 with 768 bins (default for equal masses)
+ *                  1: 512 threads, up to 2048;  2: 256 threads, up to 1024;
+ *                  3: 512 threads, up to 1024;  4: 512 threads, up to 2048
+ *                  with 1024 position bins;  5: 512 threads, up to 3072
+ *                  with 1024 bins;  6: 768 threads, up to 3072 with 768
+ *                  bins;  7: 256 threads, up to 2048 with 1024 bins;
+ *                  8: 256 threads, up to 1536 with 768 bins;  9: 256
+ *                  threads, up to 1792 with 1024 bins (the default)
  *   bucket_target  sheets per value bucket, WDY_BUCKET_MIN .. the shape's
- *                  capacity; 0 picks 4/5 of the capacity
+ *                  capacity; 0 picks 6/7 of the capacity
  *   window         neighbouring buckets a sheet may move into per step
- *                  (0 .. WDY_WINDOW_MAX, default 8; at most 31 for shape 6,
- *                  63 for shapes 3-5); farther movers travel through an
- *                  overflow list
+ *                  (0 .. WDY_WINDOW_MAX, default 6, widened at the first
+ *                  interval when sampled velocities need more; at most 31
+ *                  for shape 6, 63 for shapes 3-5); farther movers travel
+ *                  through an overflow list
  * Takes effect from the next output interval, which runs on the radix path
  * to rebuild the bucket layout.  No reference counterpart (the reference
  * re-sorts from scratch each step). */

@@ -157,7 +157,7 @@ def test_warm_unequal_deterministic(gpu_lib):
     assert bits_equal(res[0][0], res[1][0]) and bits_equal(res[0][1], res[1][1])
 
 
-@pytest.mark.parametrize('shape', [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize('shape', [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize('mass', ['equal', 'lognormal'])
 def test_warm_every_shape(gpu_lib, shape, mass):
     """Every tile shape (threads, bucket capacity, position bins) gives the

@@ -268,7 +268,7 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     bool warm = !s->small && !(flags & WDY_NO_WARM);
     for (int64_t i = 0; warm && i < n; ++i) warm = m_id[i] >= 0.0;
     if (warm) {
-      r = warm_configure(s, eq ? WARM_SHAPE_EQ : WARM_SHAPE_UNEQ, 0u, 8u);
+      r = warm_configure(s, WARM_SHAPE_DEFAULT, 0u, (u32)WARM_WINDOW_DEFAULT);
       if (r) return r;
       s->disp_rate = displacement_rate(n, x_slot, v_id, id_slot);
     }

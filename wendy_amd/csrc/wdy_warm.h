@@ -8,7 +8,7 @@ namespace wdy {
 constexpr int WARM_WMAX = 64;          // largest window (buckets either side)
 // Tile shapes the warm kernel is compiled for: threads per workgroup and the
 // most sheets one bucket may hold (its LDS capacity).  The default bucket
-// target is 3/4 of the capacity.
+// target is 6/7 of the capacity (warm_default_target).
 struct WarmShape {
   int threads, cap, nq;
   // widest window the partition's per-wave segment counters (nq u32 shared
@@ -18,17 +18,15 @@ struct WarmShape {
     return w < WARM_WMAX ? w : WARM_WMAX;
   }
 };
-constexpr int WARM_NSHAPES = 9;
-// default tile shapes (measured at N = 2^24 with the pipelined kernel), 256
-// threads per workgroup: buckets of up to 1536 sheets for equal masses
-// (three workgroups per CU), 1024 for unequal ones (four per CU).  When
-// sheets cross many buckets per step the window fitting moves to larger
-// buckets (1536, then 4096 sheets).
-constexpr int WARM_SHAPE_EQ = 8;
-constexpr int WARM_SHAPE_UNEQ = 2;
-constexpr int WARM_SHAPE_DEFAULT = WARM_SHAPE_UNEQ;
-constexpr int WARM_SHAPE_MID = 8;
+constexpr int WARM_NSHAPES = 10;
+// default tile shape (measured at N = 2^24 with the pipelined kernel, equal
+// and unequal masses): 256 threads per workgroup, buckets of up to 1792
+// sheets (three workgroups per CU), target 1536, window 6.  When sheets
+// cross many buckets per step the window fitting moves to 4096-sheet buckets.
+constexpr int WARM_SHAPE_DEFAULT = 9;
+constexpr int WARM_WINDOW_DEFAULT = 6;
 const WarmShape &warm_shape(int v);
+inline u32 warm_default_target(int cap) { return (u32)cap / 7u * 6u; }
 constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry
 constexpr u32 WARM_T_MIN = 32;
 

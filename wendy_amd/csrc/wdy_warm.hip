@@ -944,7 +944,7 @@ int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
 static const WarmShape kShapes[WARM_NSHAPES] = {
     {1024, 4096, 4096}, {512, 2048, 2048}, {256, 1024, 1024}, {512, 1024, 1024},
     {512, 2048, 1024}, {512, 3072, 1024}, {768, 3072, 768},
-    {256, 2048, 1024}, {256, 1536, 768}};
+    {256, 2048, 1024}, {256, 1536, 768}, {256, 1792, 1024}};
 
 const WarmShape &warm_shape(int v) { return kShapes[v]; }
 
@@ -959,7 +959,8 @@ int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
     case 5: return launch_shape<Shape<512, 3072, 1024>>(st, a, equal_mass, harm, mode);
     case 6: return launch_shape<Shape<768, 3072, 768>>(st, a, equal_mass, harm, mode);
     case 7: return launch_shape<Shape<256, 2048, 1024>>(st, a, equal_mass, harm, mode);
-    default: return launch_shape<Shape<256, 1536, 768>>(st, a, equal_mass, harm, mode);
+    case 8: return launch_shape<Shape<256, 1536, 768>>(st, a, equal_mass, harm, mode);
+    default: return launch_shape<Shape<256, 1792, 1024>>(st, a, equal_mass, harm, mode);
   }
 }
 
@@ -1076,7 +1077,7 @@ int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
   const u32 n = (u32)s->n;
   if (shape >= 0) w.shape = shape;
   w.cap = (u32)warm_shape(w.shape).cap;
-  if (T == 0) T = w.cap / 5 * 4;  // default fill: 4/5 of the capacity
+  if (T == 0) T = warm_default_target(w.cap);
   if (T > w.cap) T = w.cap;
   w.T = T;
   w.W = W;
@@ -1219,14 +1220,14 @@ int warm_fit(wdy_sim *s, double h) {
   w.fitted = true;
   const double need = s->disp_rate * std::fabs(h);  // ranks per step, fast tail
   if (!(need > 0.0) || !std::isfinite(need)) return WDY_OK;
-  // the current (default) shape, then larger buckets: the first whose window
-  // stays narrow (<= 12 buckets); else the largest buckets with the window
-  // they need
-  const int order[3] = {w.shape, WARM_SHAPE_MID, 0};
-  for (int k = 0; k < 3; ++k) {
+  // the current (default) shape, then the largest buckets: the current one
+  // if its window stays narrow (<= 12 buckets), else the largest buckets with
+  // the window they need
+  const int order[2] = {w.shape, 0};
+  for (int k = 0; k < 2; ++k) {
     const int sh = order[k];
     if (k > 0 && warm_shape(sh).cap <= warm_shape(order[k - 1]).cap) continue;
-    const u32 T = (u32)warm_shape(sh).cap / 5 * 4;
+    const u32 T = warm_default_target(warm_shape(sh).cap);
     const u32 W = (u32)std::ceil(need / (double)T) + 1u;
     if (W <= 12u || sh == 0) {
       const u32 wmax = (u32)warm_shape(sh).wmax();

@@ -102,7 +102,7 @@ class Sim:
                                          ctypes.byref(e)))
         return e.value
 
-    def tune(self, bucket_target=0, window=8, shape=-1):
+    def tune(self, bucket_target=0, window=6, shape=-1):
         """Warm-start tile shape, bucket target and window (large path);
         applies from the next output interval (see wdy_tune)."""
         _lib.check(_lib.lib().wdy_tune(self._h, int(shape), int(bucket_target),
@@ -115,7 +115,7 @@ class Sim:
                                              ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
-    def autotune(self, h, nleap, candidates=((2, 8), (8, 8), (0, 8)), trials=2):
+    def autotune(self, h, nleap, candidates=((9, 6), (2, 8), (0, 8)), trials=2):
         """Pick the warm-start tile shape by timing: each candidate (shape,
         window) runs ``trials`` warm output intervals of (h, nleap) on a
         temporary copy of the current state; the fastest is applied to this
