@@ -121,10 +121,11 @@ int wdy_destroy(wdy_sim *sim);
  *                  with 1024 position bins;  5: 512 threads, up to 3072
  *                  with 1024 bins;  6: 768 threads, up to 3072 with 768
  *                  bins;  7: 256 threads, up to 2048 with 1024 bins;
- *                  8: 256 threads, up to 1536 with 768 bins;  9: 256
- *                  threads, up to 1792 with 1024 bins (the default)
+ *                  8: 256 threads, up to 1536 with 768 bins (the
+ *                  default);  9: 256 threads, up to 1792 with 1024 bins
  *   bucket_target  sheets per value bucket, WDY_BUCKET_MIN .. the shape's
- *                  capacity; 0 picks 6/7 of the capacity
+ *                  capacity; 0 leaves 1/5 of the capacity (at least 256
+ *                  sheets) free
  *   window         neighbouring buckets a sheet may move into per step
  *                  (0 .. WDY_WINDOW_MAX, default 6, widened at the first
  *                  interval when sampled velocities need more; at most 31

@@ -78,10 +78,13 @@ __device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
 // One tile shape: threads per workgroup and the most sheets a bucket holds.
 // WT threads, CAP sheets per bucket, NQ position bins for the bucket sort
 // (also the per-wave segment counters of the warm step's partition).
-template <int WT_, int CAP_, int NQ_ = CAP_>
+// OCC_EQ: waves per SIMD the equal-mass pipelined kernel is compiled for
+// (its VGPR budget: 512 / OCC_EQ); 1 leaves the register count to the
+// compiler.
+template <int WT_, int CAP_, int NQ_ = CAP_, int OCC_EQ_ = 1>
 struct Shape {
   static constexpr int WT = WT_, CAP = CAP_, NQ = NQ_, PER = CAP_ / WT_;
-  static constexpr int NWAVE = WT_ / 64;
+  static constexpr int NWAVE = WT_ / 64, OCC_EQ = OCC_EQ_;
   static_assert(CAP_ % WT_ == 0 && NQ_ % WT_ == 0 && WT_ % 64 == 0 &&
                     CAP_ <= 65535, "shape");
 };
@@ -91,6 +94,7 @@ struct Lds {
   static constexpr int CAP = SH::CAP;
   static constexpr size_t BYTES =
       (size_t)CAP * (EQ ? 22 : 30) + (size_t)SH::NQ * 4;
+  static_assert(BYTES % 8 == 0, "what follows the arrays stays 8-byte aligned");
   u64 *key;
   double *v;
   double *m;  // unequal masses only

@@ -8,7 +8,10 @@ namespace wdy {
 constexpr int WARM_WMAX = 64;          // largest window (buckets either side)
 // Tile shapes the warm kernel is compiled for: threads per workgroup and the
 // most sheets one bucket may hold (its LDS capacity).  The default bucket
-// target is 6/7 of the capacity (warm_default_target).
+// target leaves 1/5 of the capacity, at least 256 sheets, as headroom
+// (warm_default_target): bucket counts drift with the net flow of sheets
+// across the stale splitters; over 2400 steps of the N = 2^24 disk, 1536-
+// sheet buckets never overfilled at targets up to 1260 and did at 1280.
 struct WarmShape {
   int threads, cap, nq;
   // widest window the partition's per-wave segment counters (nq u32 shared
@@ -20,13 +23,17 @@ struct WarmShape {
 };
 constexpr int WARM_NSHAPES = 10;
 // default tile shape (measured at N = 2^24 with the pipelined kernel, equal
-// and unequal masses): 256 threads per workgroup, buckets of up to 1792
-// sheets (three workgroups per CU), target 1536, window 6.  When sheets
-// cross many buckets per step the window fitting moves to 4096-sheet buckets.
-constexpr int WARM_SHAPE_DEFAULT = 9;
+// and unequal masses): 256 threads per workgroup, buckets of up to 1536
+// sheets, target 1229, window 6; four workgroups per CU with equal masses
+// (38 KB of LDS, 128 VGPRs), three with unequal ones.  When sheets cross
+// many buckets per step the window fitting moves to 4096-sheet buckets.
+constexpr int WARM_SHAPE_DEFAULT = 8;
 constexpr int WARM_WINDOW_DEFAULT = 6;
 const WarmShape &warm_shape(int v);
-inline u32 warm_default_target(int cap) { return (u32)cap / 7u * 6u; }
+inline u32 warm_default_target(int cap) {
+  const u32 c = (u32)cap, room = c / 5u > 256u ? c / 5u : 256u;
+  return c > 2u * room ? c - room : c / 2u;
+}
 constexpr u32 WARM_OVF_CAP = 1u << 16; // far movers one step may carry
 constexpr u32 WARM_T_MIN = 32;
 

@@ -476,26 +476,42 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
   WARM_STAMP(5);
 }
 
+// LDS bytes of the pipelined kernel's tables for S segments, placed after
+// the bucket arrays: two splitter windows (S+1 u64 each: the current
+// bucket's, read by its partition, and the next one's), then one segment
+// table (S sources, S+1 prefixes), the partition's S+1 output prefixes and
+// LOVF overflow indices.  Only the splitter window is needed by two buckets
+// at once: the next bucket's segment table and overflow indices are read
+// when its loads are issued, before the current bucket's partition, and
+// again (over capacity) at its own commit, before the following table
+// overwrites them.
+__host__ __device__ constexpr size_t pipe_table_bytes(u32 S) {
+  return 16ull * (S + 1) + 4ull * (S + 2 * (S + 1) + LOVF);
+}
+
 // Persistent, software-pipelined form of mode 0.  One workgroup per CU loops
 // over tickets; while it sorts, looks back and writes bucket j, the loads of
 // the next bucket are in flight: its segment table and splitter window are
 // requested right after bucket j is committed to LDS, and its sheets right
 // after j's look-back, so they arrive during j's force and partition.  The
-// arithmetic and the outputs are those of k_warm_step.  Tables are
-// double-buffered in LDS; each workgroup holds at most two unpublished
-// tickets, and the lowest unpublished ticket always belongs to a workgroup
-// whose own look-back waits only on lower, published buckets, so the chain
-// always advances.
+// arithmetic and the outputs are those of k_warm_step.  Each workgroup holds
+// at most two unpublished tickets, and the lowest unpublished ticket always
+// belongs to a workgroup whose own look-back waits only on lower, published
+// buckets, so the chain always advances.
 template <class SH, bool EQ, bool HARM>
-__global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
+__global__ __launch_bounds__(SH::WT)
+__attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmArgs A) {
   constexpr int WT = SH::WT, PER = SH::PER, CAP = SH::CAP, NWAVE = SH::NWAVE;
   extern __shared__ u64 lds_dyn[];
   Lds<SH, EQ> s(lds_dyn);
-  __shared__ u64 s_spl[2][SMAX + 1];
-  __shared__ u32 s_segsrc[2][SMAX];
-  __shared__ u32 s_segpre[2][SMAX + 1];
-  __shared__ u32 s_oidx[2][LOVF];
-  __shared__ u32 s_novf[2];
+  const u32 S = A.S, W = A.W, nb = A.nb;
+  u64 *const s_spl0 = lds_dyn + Lds<SH, EQ>::BYTES / 8;  // [2][S + 1]
+  u32 *const s_segsrc = (u32 *)(s_spl0 + 2 * (S + 1));   // [S]
+  u32 *const s_segpre = s_segsrc + S;                    // [S + 1]
+  u32 *const s_segout = s_segpre + S + 1;                // [S + 1]
+  u32 *const s_oidx = s_segout + S + 1;                  // [LOVF]
+  auto s_spl = [&](int bb) { return s_spl0 + (u32)bb * (S + 1); };
+  __shared__ u32 s_novf;
   __shared__ u32 s_tot;
   __shared__ u64 s_red[NWAVE];
   __shared__ u64 s_mm[2 * NWAVE];
@@ -503,13 +519,11 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
   __shared__ u64 s_u64[2];
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const u32 S = A.S, W = A.W, nb = A.nb;
   const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
   if (t == 0) {
     s_u[0] = atomicAdd(A.tick, 1u);
     s_u[1] = ld_u32(A.err);
-    s_novf[0] = 0;
-    s_novf[1] = 0;
+    s_novf = 0;
   }
   const u32 novf_in = ld_u32(A.ovf_r);
   const u32 novf_all = novf_in > WARM_OVF_CAP ? WARM_OVF_CAP : novf_in;
@@ -538,36 +552,37 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
     }
     if (t <= hi - lo + 1) tb_spl = A.rd.spl[lo + t];
   };
-  // stores the registers into buffer bb, collects q's overflow entries and
-  // scans the segment counts; returns the segment sheets
+  // stores the registers into the segment table and splitter window bb,
+  // collects q's overflow entries and scans the segment counts; returns the
+  // segment sheets
   auto table_commit = [&](u32 q, int bb) -> u32 {
     const u32 lo = q >= W ? q - W : 0u, hi = q + W < nb ? q + W : nb - 1;
     if (t < S) {
-      s_segsrc[bb][t] = tb_start;
-      s_segpre[bb][t] = tb_cnt;
+      s_segsrc[t] = tb_start;
+      s_segpre[t] = tb_cnt;
     }
-    if (t <= hi - lo + 1) s_spl[bb][t] = tb_spl;
+    if (t <= hi - lo + 1) s_spl(bb)[t] = tb_spl;
     for (u32 q2 = t; q2 < novf_all; q2 += WT)
       if (A.rd.odst[q2] == q) {
-        const u32 slot = atomicAdd(&s_novf[bb], 1u);
-        if (slot < (u32)LOVF) s_oidx[bb][slot] = q2;
+        const u32 slot = atomicAdd(&s_novf, 1u);
+        if (slot < (u32)LOVF) s_oidx[slot] = q2;
       }
     if (S > 64) __syncthreads();  // the long scan reads other waves' counts
-    return short_excl_u32_inplace<NWAVE>(s_segpre[bb], S, s_red, &s_tot);
+    return short_excl_u32_inplace<NWAVE>(s_segpre, S, s_red, &s_tot);
   };
 
   // data registers of the bucket being prefetched
   u64 dk[PER];
   double dv[PER], dm[PER];
   i32 di[PER];
-  auto data_issue = [&](u32 nseg, u32 c, int bb) {
+  auto data_issue = [&](u32 nseg, u32 c) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const u32 e = i * WT + t;
       if (e < c) {
         const bool seg = e < nseg;
-        const u32 pos = seg ? seg_pos(s_segpre[bb], s_segsrc[bb], S, e)
-                            : s_oidx[bb][e - nseg];
+        const u32 pos = seg ? seg_pos(s_segpre, s_segsrc, S, e)
+                            : s_oidx[e - nseg];
         dk[i] = (seg ? A.kin : A.rd.okey)[pos];
         dv[i] = (seg ? A.vin : A.rd.ov)[pos];
         di[i] = (seg ? A.iin : A.rd.oid)[pos];
@@ -582,10 +597,10 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
   if (j < nb) {  // prologue: bucket j's tables and loads
     table_issue(j);
     nseg = table_commit(j, b);
-    const u32 novf = s_novf[b];
+    const u32 novf = s_novf;
     c = nseg + novf;
     ok = c <= (u32)CAP && novf <= (u32)LOVF;
-    if (ok) data_issue(nseg, c, b);
+    if (ok) data_issue(nseg, c);
   }
 
   while (j < nb) {
@@ -613,7 +628,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
     } else if (!EQ) {
       // over capacity: still publish the exact mass total
       for (u32 e = t; e < nseg; e += WT)
-        msum += mass_fix(A.min[seg_pos(s_segpre[b], s_segsrc[b], S, e)], A.fscale);
+        msum += mass_fix(A.min[seg_pos(s_segpre, s_segsrc, S, e)], A.fscale);
       for (u32 q2 = t; q2 < novf_all; q2 += WT)
         if (A.rd.odst[q2] == j) msum += mass_fix(A.rd.om[q2], A.fscale);
     }
@@ -629,7 +644,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
     WARM_STAMP(1);
     const u32 jn = s_u[0];
     const int bn = b ^ 1;
-    if (t == 0) s_novf[bn] = 0;
+    if (t == 0) s_novf = 0;
     if (jn < nb) table_issue(jn);  // in flight during the sort
 
     // ---- sort, look-back ---------------------------------------------------------
@@ -714,7 +729,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
     bool ok_n = false;
     if (jn < nb) {
       nseg_n = table_commit(jn, bn);
-      const u32 novf_n = s_novf[bn];
+      const u32 novf_n = s_novf;
       c_n = nseg_n + novf_n;
       ok_n = c_n <= (u32)CAP && novf_n <= (u32)LOVF;
     }
@@ -764,13 +779,13 @@ __global__ __launch_bounds__(SH::WT) void k_warm_pipe(WarmArgs A) {
       __syncthreads();
     }
     // ---- the next bucket's loads, in flight during the partition --------------------
-    if (ok_n) data_issue(nseg_n, c_n, bn);
+    if (ok_n) data_issue(nseg_n, c_n);
     WARM_STAMP(4);
     if (write) {
       // ---- partition by next-step bucket and write out ---------------------------
       u32 *wc = s.bins;  // [NWAVE][S + 1] per-wave segment counters
-      u32 *segp = s_segpre[b];
-      const u64 *spl = s_spl[b];
+      u32 *segp = s_segout;
+      const u64 *spl = s_spl(b);
       for (u32 q = t; q < (u32)NWAVE * (S + 1); q += WT) wc[q] = 0;
       __syncthreads();
       u32 dd[PER], rr[PER];
@@ -889,22 +904,24 @@ int launch_one(hipStream_t st, const WarmArgs &a) {
 template <class SH, bool EQ, bool HARM>
 int launch_pipe(hipStream_t st, const WarmArgs &a) {
   auto fn = k_warm_pipe<SH, EQ, HARM>;
-  const size_t lds = Lds<SH, EQ>::BYTES;
-  static std::atomic<int> resident[64];  // per device; 0: not computed yet
+  const size_t lds = Lds<SH, EQ>::BYTES + pipe_table_bytes(a.S);
+  // per device: (LDS bytes << 32) | resident workgroups; 0: not computed yet
+  static std::atomic<u64> resident[64];
   if (lds > 65536)
     WDY_HIP(hipFuncSetAttribute((const void *)fn,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
   int dev = 0;
   WDY_HIP(hipGetDevice(&dev));
-  int res = dev < 64 ? resident[dev].load() : 0;
+  const u64 cached = dev < 64 ? resident[dev].load() : 0ull;
+  int res = (cached >> 32) == lds ? (int)(u32)cached : 0;
   if (res == 0) {
     int cus = 0, per = 0;
     WDY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     WDY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)fn,
                                                          SH::WT, lds));
     res = (per > 0 ? per : 1) * (cus > 0 ? cus : 1);
-    if (dev < 64) resident[dev].store(res);
+    if (dev < 64) resident[dev].store(((u64)lds << 32) | (u32)res);
   }
   const u32 grid = a.nb < (u32)res ? a.nb : (u32)res;
   fn<<<grid, SH::WT, lds, st>>>(a);
@@ -959,7 +976,7 @@ int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
     case 5: return launch_shape<Shape<512, 3072, 1024>>(st, a, equal_mass, harm, mode);
     case 6: return launch_shape<Shape<768, 3072, 768>>(st, a, equal_mass, harm, mode);
     case 7: return launch_shape<Shape<256, 2048, 1024>>(st, a, equal_mass, harm, mode);
-    case 8: return launch_shape<Shape<256, 1536, 768>>(st, a, equal_mass, harm, mode);
+    case 8: return launch_shape<Shape<256, 1536, 768, 4>>(st, a, equal_mass, harm, mode);
     default: return launch_shape<Shape<256, 1792, 1024>>(st, a, equal_mass, harm, mode);
   }
 }
@@ -1305,7 +1322,7 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     }
     u32 T = w.T, W = w.W;
     int shape = w.shape;
-    if (kind & WARM_ERR_CAPACITY) T = T * 7 / 8 < WARM_T_MIN ? WARM_T_MIN : T * 7 / 8;
+    if (kind & WARM_ERR_CAPACITY) T = T * 15 / 16 < WARM_T_MIN ? WARM_T_MIN : T * 15 / 16;
     if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) {
       // sheets cross more buckets per step than the window holds: widen it,
       // and once it is at its limit move to the largest buckets

@@ -115,7 +115,7 @@ class Sim:
                                              ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
-    def autotune(self, h, nleap, candidates=((9, 6), (2, 8), (0, 8)), trials=2):
+    def autotune(self, h, nleap, candidates=((8, 6), (9, 6), (0, 8)), trials=2):
         """Pick the warm-start tile shape by timing: each candidate (shape,
         window) runs ``trials`` warm output intervals of (h, nleap) on a
         temporary copy of the current state; the fastest is applied to this
