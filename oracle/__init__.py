@@ -35,6 +35,9 @@ term, a wrong sign or index, or a transposed operand fails a test):
   term); and against the momentum change a uniform time-dependent force must
   produce, M h sum_k g(t0 + k h), which a half-step clock error fails.
 * ``canonical_order`` -- against brute-force ranking of tiny inputs.
+* ``canonical_order_presorted`` -- equal to ``canonical_order`` on tie-free,
+  tied (-0.0 vs +0.0 included), random and nearly sorted inputs, and a whole
+  ``ApproxState`` trajectory is bitwise the same with either.
 * ``force_sorted`` -- against the brute-force pair sum
   ``sum_j m_j sign(x_j - x_i)`` with ``math.fsum``.
 * ``energy`` / ``momentum`` / ``potential`` / ``energy_individual`` --

@@ -160,6 +160,30 @@ def test_large_n_bitwise_against_oracle(gpu_lib):
     assert_parity(got[0], st.onestep(), True, 'n=2^21')
 
 
+def test_headline_workload_bitwise_at_full_size(gpu_lib):
+    """N = 2^24, equal masses 2^-24, omega = 0.1, h = 0.005 / 20 (the
+    single-GPU workload at its real size): a cold output interval on the
+    radix path, then a warm-start one (first, middle and last step forms),
+    each equal to the oracle bit for bit."""
+    n = 1 << 24
+    h, nleap = 0.005 / 20, 3
+    print('headline 2^24: generating', flush=True)
+    x, v, m = O.sech2_disk(n, seed=0)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.1 ** 2.)
+    xo, vo = numpy.empty(n), numpy.empty(n)
+    st = O.ApproxState(x, v, m, h * nleap, nleap, omega=0.1,
+                       presorted_order=True)
+    try:
+        for k in range(2):
+            sim.advance(h, nleap)
+            sim.read(xo, vo)
+            print('headline 2^24: oracle interval', k + 1, flush=True)
+            assert_parity((xo, vo), st.onestep(), True, 'output %d' % (k + 1))
+        assert sim.path_stats() == (nleap, nleap, 0)
+    finally:
+        sim.close()
+
+
 @pytest.mark.parametrize('n,path', [(3000, None), (50000, 'large')])
 def test_read_and_advance_equals_step_by_step(gpu_lib, n, path):
     """The overlapped generator step (de-sort, start the next interval, copy

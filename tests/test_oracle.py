@@ -55,6 +55,29 @@ def test_canonical_order_brute_force():
             assert r == pos
 
 
+def test_presorted_order_equals_canonical_order():
+    rs = numpy.random.RandomState(2)
+    for n in (1, 2, 7, 500, 20000):
+        idx = rs.permutation(n).astype(numpy.int32)
+        val = rs.normal(size=n)
+        near = numpy.sort(val) + 1e-3 * rs.normal(size=n)   # a few swaps
+        ties = rs.choice([-1.0, -0.0, 0.0, 3.0], size=n)
+        for a in (val, near, ties):
+            assert numpy.array_equal(O.canonical_order_presorted(a, idx),
+                                     O.canonical_order(a, idx))
+
+
+def test_presorted_order_gives_the_same_trajectory():
+    x, v, m = O.sech2_disk(3000, seed=5, mass=('lognormal', 0.5))
+    x[7] = x[8]                      # an exact tie at the start
+    a = O.ApproxState(x, v, m, 0.01, 4, omega=0.2)
+    b = O.ApproxState(x, v, m, 0.01, 4, omega=0.2, presorted_order=True)
+    for _ in range(3):
+        ax, av = a.onestep()
+        bx, bv = b.onestep()
+        assert bits_equal(ax, bx) and bits_equal(av, bv)
+
+
 @pytest.mark.parametrize('omega', [None, 1.3])
 def test_force_matches_brute_force_pair_sum(omega):
     rs = numpy.random.RandomState(4)
