@@ -111,10 +111,12 @@ struct Lds {
   }
 };
 
-// s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
+// rk[i] := the rank, in the (key, id) order of s[0..c), of sheet i * WT + t
+// (for i * WT + t < c).  s.ord holds the bins' contents on return; a barrier
+// must separate the ranking's LDS reads from later writes to s.
 template <class SH, bool EQ>
-__device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
-                                u64 *prof) {
+__device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
+                                u64 *prof, u32 (&rk)[SH::PER]) {
   constexpr int WT = SH::WT, PER = SH::PER, NQ = SH::NQ, NWAVE = SH::NWAVE;
   const u32 t = threadIdx.x;
   u64 kmin = ~0ull, kmax = 0ull;
@@ -206,8 +208,7 @@ __device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
   if (prof && t == 0) prof[7] = wall_clock64();
   // bins[b] is now the end of bin b.  Rank each sheet inside its bin by
   // counting the bin's smaller (key, id) pairs -- every sheet in parallel, a
-  // bin of k sheets costs k comparisons per sheet -- then place it.
-  u32 rk[PER];
+  // bin of k sheets costs k comparisons per sheet.
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const u32 e = i * WT + t;
@@ -230,6 +231,16 @@ __device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
       rk[i] = r;
     }
   }
+}
+
+// s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
+template <class SH, bool EQ>
+__device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
+                                u64 *prof) {
+  constexpr int WT = SH::WT, PER = SH::PER;
+  const u32 t = threadIdx.x;
+  u32 rk[PER];
+  lds_bucket_rank<SH, EQ>(s, c, s_red, s_mm, prof, rk);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < PER; ++i) {

@@ -648,9 +648,21 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     if (jn < nb) table_issue(jn);  // in flight during the sort
 
     // ---- sort, look-back ---------------------------------------------------------
-    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm,
-                                    A.prof ? A.prof + 8ull * j : nullptr);
-    else __syncthreads();
+    // equal masses need only each sheet's rank (the force is a function of
+    // it); the sheets stay where their threads put them, and the force and
+    // the partition read them there.  Unequal masses need the sorted order
+    // for the mass prefix.
+    u32 rk[PER];
+    if (ok) {
+      if constexpr (EQ)
+        lds_bucket_rank<SH, EQ>(s, c, s_red, s_mm,
+                                A.prof ? A.prof + 8ull * j : nullptr, rk);
+      else
+        lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm,
+                                A.prof ? A.prof + 8ull * j : nullptr);
+    } else {
+      __syncthreads();
+    }
     WARM_STAMP(2);
     if (w == 0) {
       u64 ecnt = 0, emass = 0;
@@ -734,7 +746,32 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       ok_n = c_n <= (u32)CAP && novf_n <= (u32)LOVF;
     }
 
-    if (write) {
+    if (write && EQ) {
+      // ---- force, kick, drift of each thread's own sheets, by rank ------------------
+      // (the look-back's barrier separates the ranking's reads from these
+      // writes); the splitters are the sheets whose ranks are multiples of T
+      const u32 sfirst = (off + A.T - 1) / A.T, send = off + c;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const u32 e = i * WT + t;
+        if (e < c) {
+          const u32 R = off + rk[i];
+          const u64 kold = s.key[e];
+          double x = decode_pos(kold);
+          double F = (A.totmass - 2.0 * ((double)R * A.mq)) - A.mq;
+          if (HARM) F = F - A.omega2 * x;
+          const double a = (A.ext.kind ? ext_accel(A.ext, x, A.t) : 0.0) + F;
+          const double v = s.v[e] + A.h * a;
+          x = x + A.hd * v;
+          s.v[e] = v;
+          s.key[e] = encode_pos(x);
+          for (u32 si = sfirst; si * A.T < send; ++si)
+            if (R == si * A.T) A.wr.spl[si] = R == 0 ? 0ull : kold;
+        }
+      }
+      if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
+      __syncthreads();
+    } else if (write) {
       // ---- force, kick, drift in sorted order ------------------------------------
       {
         const u32 r0 = t * PER;
@@ -795,7 +832,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         const bool valid = r < c;
         u32 d = S;
         if (valid) {
-          const u32 e = s.ord[r];
+          const u32 e = EQ ? r : (u32)s.ord[r];
           u64 kp = s.key[e];
           if (A.hp != 0.0) kp = encode_pos(decode_pos(kp) + A.hp * s.v[e]);
           const bool below = lo > 0 && kp < spl[0];
@@ -858,7 +895,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       for (int i = 0; i < PER; ++i) {
         const u32 r = i * WT + t;
         if (r < c) {
-          const u32 e = s.ord[r];
+          const u32 e = EQ ? r : (u32)s.ord[r];
           const u32 p = off + segp[dd[i]] + wc[w * (S + 1) + dd[i]] + rr[i];
           A.kout[p] = s.key[e];
           A.vout[p] = s.v[e];
