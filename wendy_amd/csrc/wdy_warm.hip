@@ -498,7 +498,9 @@ __host__ __device__ constexpr size_t pipe_table_bytes(u32 S) {
 // at most two unpublished tickets, and the lowest unpublished ticket always
 // belongs to a workgroup whose own look-back waits only on lower, published
 // buckets, so the chain always advances.
-template <class SH, bool EQ, bool HARM>
+// EXT: the simulation has an external force (A.ext); without one the kernel
+// carries none of its code, which keeps the register count down.
+template <class SH, bool EQ, bool HARM, bool EXT>
 __global__ __launch_bounds__(SH::WT)
 __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmArgs A) {
   constexpr int WT = SH::WT, PER = SH::PER, CAP = SH::CAP, NWAVE = SH::NWAVE;
@@ -760,7 +762,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           double x = decode_pos(kold);
           double F = (A.totmass - 2.0 * ((double)R * A.mq)) - A.mq;
           if (HARM) F = F - A.omega2 * x;
-          const double a = (A.ext.kind ? ext_accel(A.ext, x, A.t) : 0.0) + F;
+          const double a = (EXT ? ext_accel(A.ext, x, A.t) : 0.0) + F;
           const double v = s.v[e] + A.h * a;
           x = x + A.hd * v;
           s.v[e] = v;
@@ -803,7 +805,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
               run += mass_fix(me, A.fscale);
             }
             if (HARM) F = F - A.omega2 * x;
-            const double a = (A.ext.kind ? ext_accel(A.ext, x, A.t) : 0.0) + F;
+            const double a = (EXT ? ext_accel(A.ext, x, A.t) : 0.0) + F;
             const double v = s.v[e] + A.h * a;
             x = x + A.hd * v;
             s.v[e] = v;
@@ -938,9 +940,9 @@ int launch_one(hipStream_t st, const WarmArgs &a) {
 
 // Workgroups of the persistent mode-0 kernel: as many as are resident at
 // once (more would only find the tickets exhausted).
-template <class SH, bool EQ, bool HARM>
+template <class SH, bool EQ, bool HARM, bool EXT>
 int launch_pipe(hipStream_t st, const WarmArgs &a) {
-  auto fn = k_warm_pipe<SH, EQ, HARM>;
+  auto fn = k_warm_pipe<SH, EQ, HARM, EXT>;
   const size_t lds = Lds<SH, EQ>::BYTES + pipe_table_bytes(a.S);
   // per device: (LDS bytes << 32) | resident workgroups; 0: not computed yet
   static std::atomic<u64> resident[64];
@@ -971,10 +973,14 @@ int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
                  int mode) {
   if (mode == 2) {  // mode 0, persistent pipelined kernel
     if (eq)
-      return harm ? launch_pipe<SH, true, true>(st, a)
-                  : launch_pipe<SH, true, false>(st, a);
-    return harm ? launch_pipe<SH, false, true>(st, a)
-                : launch_pipe<SH, false, false>(st, a);
+      return a.ext.kind ? (harm ? launch_pipe<SH, true, true, true>(st, a)
+                                : launch_pipe<SH, true, false, true>(st, a))
+                        : (harm ? launch_pipe<SH, true, true, false>(st, a)
+                                : launch_pipe<SH, true, false, false>(st, a));
+    return a.ext.kind ? (harm ? launch_pipe<SH, false, true, true>(st, a)
+                              : launch_pipe<SH, false, false, true>(st, a))
+                      : (harm ? launch_pipe<SH, false, true, false>(st, a)
+                              : launch_pipe<SH, false, false, false>(st, a));
   }
   if (mode == 0) {
     if (eq)
