@@ -602,6 +602,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     const u32 novf = s_novf;
     c = nseg + novf;
     ok = c <= (u32)CAP && novf <= (u32)LOVF;
+    // equal masses: the aggregate is the count alone, known from the table
+    if (EQ && t == 0) publish(A.status + STATUS_WORDS * j, ep, j == 0 ? 2u : 1u, c, 0, false);
     if (ok) data_issue(nseg, c);
   }
 
@@ -637,7 +639,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     u64 mtot = 0;
     if (!EQ) (void)block_excl_u64<NWAVE>(msum, s_red, &mtot);
     if (t == 0) {
-      publish(A.status + STATUS_WORDS * j, ep, j == 0 ? 2u : 1u, c, mtot, !EQ);
+      // (equal masses published the count with the table)
+      if (!EQ) publish(A.status + STATUS_WORDS * j, ep, j == 0 ? 2u : 1u, c, mtot, true);
       if (!ok)
         set_err(A.err, A.epoch,
                 c - nseg > (u32)LOVF ? WARM_ERR_LOVF : WARM_ERR_CAPACITY);
@@ -746,6 +749,10 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       const u32 novf_n = s_novf;
       c_n = nseg_n + novf_n;
       ok_n = c_n <= (u32)CAP && novf_n <= (u32)LOVF;
+      // equal masses: publish the next bucket's count now, half an iteration
+      // before its commit, so that later buckets' look-backs wait less
+      if (EQ && t == 0)
+        publish(A.status + STATUS_WORDS * jn, ep, jn == 0 ? 2u : 1u, c_n, 0, false);
     }
 
     if (write && EQ) {
