@@ -160,6 +160,26 @@ def test_large_n_bitwise_against_oracle(gpu_lib):
     assert_parity(got[0], st.onestep(), True, 'n=2^21')
 
 
+def test_generator_alive_at_interpreter_exit(gpu_lib):
+    """A process that exits with an nbody generator still open (its device
+    state and page-locked outputs alive) exits cleanly and quietly."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import numpy, wendy_amd\n"
+            "from oracle import nbody_oracle as O\n"
+            "x, v, m = O.sech2_disk(20000, seed=4)\n"
+            "g = wendy_amd.nbody(x, v, m, 0.01, approx=True, nleap=4)\n"
+            "next(g); next(g)\n"
+            "print('done', flush=True)\n")
+    r = subprocess.run([sys.executable, '-c', code], cwd=root, timeout=100,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert 'done' in r.stdout
+    assert 'Exception' not in r.stderr and 'Error' not in r.stderr, r.stderr[-2000:]
+
+
 def test_headline_workload_bitwise_at_full_size(gpu_lib):
     """N = 2^24, equal masses 2^-24, omega = 0.1, h = 0.005 / 20 (the
     single-GPU workload at its real size): a cold output interval on the

@@ -8,6 +8,7 @@ ImportError-like ``RuntimeError`` immediately on first use: there is no CPU
 fallback.
 """
 import ctypes
+import sys
 import os
 
 import numpy
@@ -201,15 +202,19 @@ class _HostBlock:
 
     def __init__(self, nbytes):
         p = ctypes.c_void_p()
+        self.ptr = None
+        self._free = lib().wdy_host_free
         check(lib().wdy_host_alloc(int(nbytes), ctypes.byref(p)))
         self.ptr = p.value
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the memory itself
         try:
-            if self.ptr:
-                lib().wdy_host_free(self.ptr)
+            if self.ptr and not _finalizing():
+                self._free(self.ptr)
         except Exception:
             pass
+        self.ptr = None
         self.ptr = None
 
 

@@ -6,6 +6,7 @@ generator); each simulation's arithmetic is that of ``wendy_amd.nbody`` with
 interval (see wdy_batch_create in include/wendy_amd.h).
 """
 import ctypes
+import sys
 
 import numpy
 
@@ -70,10 +71,11 @@ class Batch:
                                              ctypes.byref(sec)))
         return sec.value
 
-    def close(self):
-        if self._h is not None and self._h.value:
-            _lib.lib().wdy_batch_destroy(self._h)
-        self._h = None
+    def close(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the device memory itself
+        h, self._h = getattr(self, '_h', None), None
+        if h is not None and h.value and not _finalizing():
+            _lib.lib().wdy_batch_destroy(h)
 
     def __del__(self):
         try:

@@ -9,6 +9,7 @@ positions and velocities back into the generator's own ``x``, ``v`` arrays.
 """
 import copy
 import ctypes
+import sys
 import warnings
 
 import numpy
@@ -34,6 +35,7 @@ class Sim:
         the large path on the full radix sort every step."""
         self._h = None
         lib = _lib.lib()
+        self._destroy = lib.wdy_destroy  # held for close() at interpreter exit
         x = _lib.f64(x)
         v = _lib.f64(v)
         m = _lib.f64(m)
@@ -181,10 +183,12 @@ class Sim:
         _lib.check(_lib.lib().wdy_time(self._h, ctypes.byref(t)))
         return t.value
 
-    def close(self):
-        if self._h is not None and self._h.value:
-            _lib.lib().wdy_destroy(self._h)
-        self._h = None
+    def close(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit (a generator collected then) the process
+        # releases the device memory itself; the HIP runtime may be gone
+        h, self._h = getattr(self, '_h', None), None
+        if h is not None and h.value and not _finalizing():
+            self._destroy(h)
 
     def __del__(self):
         try:

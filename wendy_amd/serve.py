@@ -25,6 +25,7 @@ A simulation's result does not depend on what else is being served.
 import collections
 import concurrent.futures
 import ctypes
+import sys
 import time
 
 import numpy
@@ -114,10 +115,11 @@ class Pool:
                                              ctypes.byref(used), ctypes.byref(big)))
         return live.value, used.value, big.value
 
-    def close(self):
-        if self._h is not None and self._h.value:
-            _lib.lib().wdy_pool_destroy(self._h)
-        self._h = None
+    def close(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the device memory itself
+        h, self._h = getattr(self, '_h', None), None
+        if h is not None and h.value and not _finalizing():
+            _lib.lib().wdy_pool_destroy(h)
 
     def __del__(self):
         try:
