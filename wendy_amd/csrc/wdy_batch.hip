@@ -12,15 +12,19 @@
 // * n <= BLOCK (4096): the whole simulation stays in LDS for the interval;
 //   each step sorts it with the LDS bucket sort (a full sort: any input
 //   order), then computes force, kick and drift in sorted order.
-// * n > BLOCK: each step sorts the arena range in place with a block
-//   odd-even merge sort -- sort aligned blocks of BLOCK sheets in LDS, then
-//   blocks offset by BLOCK/2 -- repeated until every block boundary is in
-//   order.  Because each block sort only permutes its own block, ordered
-//   boundaries mean the range is sorted; from the previous step's order (a
-//   few ranks of disorder) one round suffices, and any input order finishes
-//   within ceil(2n/BLOCK) + 1 rounds.  A last pass computes force, kick and
-//   drift block by block, carrying the rank and the fixed-point mass prefix,
-//   and writes the other buffer set.
+// * n > BLOCK: each step is one pass of a window sliding along the previous
+//   order (slide_step): sorted in LDS, its lower part final and written out
+//   with force, kick and drift, its upper half kept for the next window.
+//   When a sheet has moved more than BLOCK/2 ranks since the previous step
+//   (the first step from an arbitrary input order, for one) the pass detects
+//   it and the step is redone in full: the arena range is sorted in place
+//   with a block odd-even merge sort -- sort aligned blocks of BLOCK sheets
+//   in LDS, then blocks offset by BLOCK/2 -- repeated until every block
+//   boundary is in order (each block sort only permutes its own block, so
+//   ordered boundaries mean the range is sorted; any input order finishes
+//   within ceil(2n/BLOCK) + 1 rounds), then a last pass computes force, kick
+//   and drift block by block, carrying the rank and the fixed-point mass
+//   prefix, and writes the other buffer set.
 //
 // Arithmetic is the reference's (wendy/wendy.c:318-333, 371-383); ties break
 // by sheet index; the unequal-mass prefix is exact fixed point (as in the
@@ -115,6 +119,83 @@ __device__ u64 block_leap(Lds<BSH, false> &s, const BatchDesc &d, u32 c,
   return run + total;
 }
 
+// One leapfrog step of a simulation with n > BLOCK sheets, from the arrays
+// key..mass (positions already drifted to the force time) into ko..mo, in
+// ONE pass: a window slides along the previous order by BLOCK/2 sheets;
+// each window is sorted in LDS, all but its upper BLOCK/2 sheets are final
+// (force, kick, drift, written out) and the upper half stays, in its LDS
+// slots, for the next window, whose new sheets fill the slots just written
+// out.  The first window holds BLOCK/2 + ((n - BLOCK) mod BLOCK/2) sheets
+// (BLOCK when that remainder is 0) so that every later one is full.  That
+// holds while no sheet has moved more than BLOCK/2 ranks since the previous
+// step; a sheet arriving below one already written shows that it did not,
+// and the step reports failure (the inputs are untouched, so the caller
+// redoes the step in full).  Returns true on success.
+__device__ bool slide_step(Lds<BSH, false> &s, const BatchDesc &d, u32 n,
+                           const u64 *key, const double *vel, const i32 *idx,
+                           const double *mass, u64 *ko, double *vo, i32 *io,
+                           double *mo, double hd, u64 *s_red, u64 *s_mm,
+                           u32 *s_ok, u64 *s_last) {
+  constexpr u32 HALF = BLOCK / 2;
+  const u32 t = threadIdx.x;
+  if (t == 0) *s_ok = 1u;
+  const u32 rem = (n - (u32)BLOCK) % HALF;
+  u32 c = rem ? HALF + rem : (u32)BLOCK;  // sheets in the current window
+  u32 lo = 0, done = 0, ne = 0;
+  bool first = true;
+  u64 run = 0;
+  while (done < n) {
+    // this window's new sheets: all c at first; then HALF of them, into the
+    // slots written out last (s.ord[0 .. ne)) and the ones the first window
+    // left unused ([c_prev, BLOCK))
+    const u32 cprev = c, cn = first ? c : HALF;
+    if (!first) c = BLOCK;
+#pragma unroll
+    for (int i = 0; i < BPER; ++i) {
+      const u32 jn = i * BWT + t;
+      if (jn < cn) {
+        const u32 e = first ? jn : (jn < ne ? (u32)s.ord[jn] : cprev + (jn - ne));
+        const u32 g = lo + jn;
+        s.key[e] = key[g];
+        s.v[e] = vel[g];
+        s.id[e] = idx[g];
+        s.m[e] = mass ? mass[g] : 0.0;
+      }
+    }
+    lo += cn;
+    __syncthreads();
+    lds_bucket_sort<BSH, false>(s, c, s_red, s_mm, nullptr);
+    ne = lo == n ? c : c - HALF;  // sheets final in this window
+    if (t == 0) {
+      const u32 e0 = s.ord[0];
+      if (!first && pair_lt(s.key[e0], s.id[e0], s_last[0], (i32)s_last[1]))
+        *s_ok = 0u;  // precedes a sheet already written
+      const u32 el = s.ord[ne - 1];
+      s_last[0] = s.key[el];
+      s_last[1] = (u64)(u32)s.id[el];
+    }
+    __syncthreads();
+    run = block_leap(s, d, ne, done, run, hd, s_red);
+#pragma unroll
+    for (int i = 0; i < BPER; ++i) {
+      const u32 r = i * BWT + t;
+      if (r < ne) {
+        const u32 e = s.ord[r];
+        ko[done + r] = s.key[e];
+        vo[done + r] = s.v[e];
+        io[done + r] = s.id[e];
+        if (mo) mo[done + r] = s.m[e];
+      }
+    }
+    __syncthreads();
+    done += ne;
+    first = false;
+  }
+  const bool ok = *s_ok != 0u;
+  __syncthreads();
+  return ok;
+}
+
 __device__ __forceinline__ bool ordered(const u64 *key, const i32 *idx,
                                         u64 a, u64 b) {
   const u64 ka = key[a], kb = key[b];
@@ -128,6 +209,7 @@ __global__ __launch_bounds__(BWT) void k_batch_advance(
   __shared__ u64 s_red[BNWAVE];
   __shared__ u64 s_mm[2 * BNWAVE];
   __shared__ u32 s_flag;
+  __shared__ u64 s_last[2];
   const BatchDesc d = descs[blockIdx.x];
   const u32 t = threadIdx.x;
   const u32 n = d.n;
@@ -191,6 +273,17 @@ __global__ __launch_bounds__(BWT) void k_batch_advance(
     double *vel = A.vel[a] + d.off;
     i32 *idx = A.idx[a] + d.off;
     double *mass = eq ? nullptr : A.mass[a] + d.off;
+    {  // one sliding pass; else the full sort below
+      const int b = a ^ 1;
+      if (slide_step(s, d, n, key, vel, idx, mass, A.key[b] + d.off,
+                     A.vel[b] + d.off, A.idx[b] + d.off,
+                     eq ? nullptr : A.mass[b] + d.off,
+                     step == d.nleap - 1 ? hh : d.h, s_red, s_mm, &s_flag,
+                     s_last)) {
+        a = b;
+        continue;
+      }
+    }
     bool sorted = false;
     for (u32 round = 0; round < max_rounds && !sorted; ++round) {
       for (u32 lo = 0; lo < n; lo += BLOCK)
