@@ -121,14 +121,15 @@ __device__ u64 block_leap(Lds<BSH, false> &s, const BatchDesc &d, u32 c,
 
 // One leapfrog step of a simulation with n > BLOCK sheets, from the arrays
 // key..mass (positions already drifted to the force time) into ko..mo, in
-// ONE pass: a window slides along the previous order by BLOCK/2 sheets;
-// each window is sorted in LDS, all but its upper BLOCK/2 sheets are final
-// (force, kick, drift, written out) and the upper half stays, in its LDS
-// slots, for the next window, whose new sheets fill the slots just written
-// out.  The first window holds BLOCK/2 + ((n - BLOCK) mod BLOCK/2) sheets
-// (BLOCK when that remainder is 0) so that every later one is full.  That
-// holds while no sheet has moved more than BLOCK/2 ranks since the previous
-// step; a sheet arriving below one already written shows that it did not,
+// ONE pass: a window of BLOCK sheets slides along the previous order by
+// STRIDE = BLOCK - KEEP sheets; each window is sorted in LDS, all but its
+// upper KEEP sheets are final (force, kick, drift, written out) and those
+// stay, in their LDS slots, for the next window, whose new sheets fill the
+// slots just written out.  The first window holds KEEP + ((n - BLOCK) mod
+// STRIDE) sheets (BLOCK when that remainder is 0) so that every later one
+// is full.  That holds while no sheet has fallen more than KEEP ranks behind
+// its place in the previous order; a sheet arriving below one already
+// written shows that it did not,
 // and the step reports failure (the inputs are untouched, so the caller
 // redoes the step in full).  Returns true on success.
 __device__ bool slide_step(Lds<BSH, false> &s, const BatchDesc &d, u32 n,
@@ -136,19 +137,19 @@ __device__ bool slide_step(Lds<BSH, false> &s, const BatchDesc &d, u32 n,
                            const double *mass, u64 *ko, double *vo, i32 *io,
                            double *mo, double hd, u64 *s_red, u64 *s_mm,
                            u32 *s_ok, u64 *s_last) {
-  constexpr u32 HALF = BLOCK / 2;
+  constexpr u32 KEEP = BLOCK / 4, STRIDE = BLOCK - KEEP;
   const u32 t = threadIdx.x;
   if (t == 0) *s_ok = 1u;
-  const u32 rem = (n - (u32)BLOCK) % HALF;
-  u32 c = rem ? HALF + rem : (u32)BLOCK;  // sheets in the current window
+  const u32 rem = (n - (u32)BLOCK) % STRIDE;
+  u32 c = rem ? KEEP + rem : (u32)BLOCK;  // sheets in the current window
   u32 lo = 0, done = 0, ne = 0;
   bool first = true;
   u64 run = 0;
   while (done < n) {
-    // this window's new sheets: all c at first; then HALF of them, into the
-    // slots written out last (s.ord[0 .. ne)) and the ones the first window
-    // left unused ([c_prev, BLOCK))
-    const u32 cprev = c, cn = first ? c : HALF;
+    // this window's new sheets: all c at first; then STRIDE of them, into
+    // the slots written out last (s.ord[0 .. ne)) and the ones the first
+    // window left unused ([c_prev, BLOCK))
+    const u32 cprev = c, cn = first ? c : STRIDE;
     if (!first) c = BLOCK;
 #pragma unroll
     for (int i = 0; i < BPER; ++i) {
@@ -165,7 +166,7 @@ __device__ bool slide_step(Lds<BSH, false> &s, const BatchDesc &d, u32 n,
     lo += cn;
     __syncthreads();
     lds_bucket_sort<BSH, false>(s, c, s_red, s_mm, nullptr);
-    ne = lo == n ? c : c - HALF;  // sheets final in this window
+    ne = lo == n ? c : c - KEEP;  // sheets final in this window
     if (t == 0) {
       const u32 e0 = s.ord[0];
       if (!first && pair_lt(s.key[e0], s.id[e0], s_last[0], (i32)s_last[1]))
