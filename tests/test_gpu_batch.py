@@ -4,9 +4,10 @@ simulation path (bitwise for power-of-two equal masses, else ||.||_inf
 relative 1e-10) -- whatever its size and whatever else shares the launch.
 
 Sizes cover both device regimes: n <= 4096 (the simulation stays in one
-workgroup's LDS for the interval) and n > 4096 (arena-resident block
-odd-even merge sort, first from the unsorted input order), with partial last
-blocks, n = 1 and exactly tied positions."""
+workgroup's LDS for the interval) and n > 4096 (arena-resident: one pass of
+a sliding sorted window per step, redone by a block odd-even merge sort when
+a sheet moved too far -- always on the first step, from the unsorted input
+order), with partial first windows, n = 1 and exactly tied positions."""
 import numpy
 import pytest
 
@@ -83,6 +84,17 @@ def test_batch_arena_resident_sizes(gpu_lib):
     specs = [(4097, 'equal', None), (6000, 'lognormal', 0.1), (8192, 'equal', 0.05),
              (10000, 'ties', None), (20000, 'lognormal', None), (33000, 'equal', None)]
     run_batch_vs_oracle(specs, 0.01, 4, 3, seed=11)
+
+
+def test_batch_large_steps_redo_mid_run(gpu_lib):
+    """Steps long enough that sheets move past the sliding window's
+    tolerance (over a thousand ranks per step at n = 20000) after some
+    outputs of short steps: the window's check fails mid-run, the step is
+    redone by the full sort, and the results still equal the oracle's; then
+    short steps again."""
+    specs = [(20000, 'equal', None), (9000, 'lognormal', 0.1), (300, 'equal', None)]
+    dt = 0.02
+    run_batch_vs_oracle(specs, dt, 4, 5, hs=[0.005, 0.005, 0.3, 0.3, 0.005])
 
 
 def test_batch_mixed_with_h_and_nleap_change(gpu_lib):
