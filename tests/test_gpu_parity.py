@@ -211,17 +211,21 @@ def test_read_and_advance_equals_step_by_step(gpu_lib, n, path):
     pageable output arrays, and the yielded arrays outlive the generator."""
     from wendy_amd import _lib
     x, v, m = O.sech2_disk(n, seed=n + 1)
-    h, nleap = 0.001, 7
+    nleap = 7
+    # the step changes after the third output: the interval started by that
+    # call runs on the radix path (the partition anticipated the old step)
+    hs = [0.001, 0.001, 0.001, 0.0013, 0.0013, 0.0013]
     ref = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path=path)
     a = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path=path)
     rx, rv = numpy.empty(n), numpy.empty(n)
     px, pv = _lib.pinned_empty(n), _lib.pinned_empty(n)
     qx, qv = numpy.empty(n), numpy.empty(n)
     try:
-        a.advance(h, nleap)
-        for k in range(4):
-            ref.advance(h, nleap)
+        a.advance(hs[0], nleap)
+        for k in range(5):
+            ref.advance(hs[k], nleap)
             ref.read(rx, rv)
+            h = hs[k + 1]          # the interval started by this call
             if k % 2 == 0:
                 a.read_and_advance(px, pv, h, nleap)
                 gx, gv = px, pv
