@@ -3,8 +3,8 @@ it gives alone -- the oracle's result, to the same bars as the single
 simulation path (bitwise for power-of-two equal masses, else ||.||_inf
 relative 1e-10) -- whatever its size and whatever else shares the launch.
 
-Sizes cover both device regimes: n <= 4096 (the simulation stays in one
-workgroup's LDS for the interval) and n > 4096 (arena-resident: one pass of
+Sizes cover both device regimes: n <= 2048 (the simulation stays in one
+workgroup's LDS for the interval) and n > 2048 (arena-resident: one pass of
 a sliding sorted window per step, redone by a block odd-even merge sort when
 a sheet moved too far -- always on the first step, from the unsorted input
 order), with partial first windows, n = 1 and exactly tied positions."""
@@ -74,15 +74,17 @@ def test_batch_lds_resident_sizes(gpu_lib):
     print('batch lds sizes', flush=True)
     specs = [(1, 'equal', None), (2, 'unequal', 0.1), (3, 'equal', 0.2),
              (64, 'equal', None), (100, 'lognormal', None), (777, 'lognormal', 0.05),
-             (1024, 'equal', 0.1), (2500, 'ties', None), (4095, 'lognormal', None),
-             (4096, 'equal', None)]
+             (1024, 'equal', 0.1), (2000, 'ties', None), (2047, 'lognormal', None),
+             (2048, 'equal', None)]
     run_batch_vs_oracle(specs, 0.01, 5, 3)
 
 
 def test_batch_arena_resident_sizes(gpu_lib):
     print('batch arena sizes', flush=True)
-    specs = [(4097, 'equal', None), (6000, 'lognormal', 0.1), (8192, 'equal', 0.05),
-             (10000, 'ties', None), (20000, 'lognormal', None), (33000, 'equal', None)]
+    specs = [(2049, 'equal', None), (2500, 'ties', None), (3585, 'lognormal', 0.1),
+             (4096, 'equal', None), (4097, 'equal', None), (6000, 'lognormal', 0.1),
+             (8192, 'equal', 0.05), (10000, 'ties', None), (20000, 'lognormal', None),
+             (33000, 'equal', None)]
     run_batch_vs_oracle(specs, 0.01, 4, 3, seed=11)
 
 
@@ -98,11 +100,12 @@ def test_batch_large_steps_redo_mid_run(gpu_lib):
 
 
 def test_batch_mixed_with_h_and_nleap_change(gpu_lib):
-    """Sizes on both sides of 4096 in one launch; h changes between outputs
+    """Sizes on both sides of 2048 in one launch; h changes between outputs
     and so does the parity of the step count (the current buffer set flips
     on the device)."""
     print('batch mixed', flush=True)
     specs = [(300, 'equal', None), (5000, 'lognormal', 0.1), (4096, 'lognormal', None),
+             (2048, 'equal', 0.1), (2049, 'lognormal', None),
              (12345, 'equal', 0.2), (17, 'lognormal', None)]
     xs, vs, ms, oms = ensemble(specs, 5)
     b = wendy_amd.Batch(xs, vs, ms, oms)

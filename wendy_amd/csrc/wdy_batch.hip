@@ -1,7 +1,7 @@
 // Batched ensemble: thousands of independent simulations of uneven N advance
 // together, ONE launch per output interval.
 //
-// Each simulation is handled by one 1024-thread workgroup that runs all of
+// Each simulation is handled by one 512-thread workgroup that runs all of
 // its leapfrog steps of the interval without returning to the host (a
 // persistent workgroup per simulation; descriptors are ordered largest
 // first so the longest workgroups start first).  State lives in one arena:
@@ -9,13 +9,13 @@
 // (key = order-preserving position encoding, velocity, global sheet index,
 // mass * twopiG); `cur` in the descriptor says which set is current.
 //
-// * n <= BLOCK (4096): the whole simulation stays in LDS for the interval;
+// * n <= BLOCK (2048): the whole simulation stays in LDS for the interval;
 //   each step sorts it with the LDS bucket sort (a full sort: any input
 //   order), then computes force, kick and drift in sorted order.
 // * n > BLOCK: each step is one pass of a window sliding along the previous
 //   order (slide_step): sorted in LDS, its lower part final and written out
 //   with force, kick and drift, its upper half kept for the next window.
-//   When a sheet has moved more than BLOCK/2 ranks since the previous step
+//   When a sheet has fallen more than BLOCK/4 ranks behind its previous place
 //   (the first step from an arbitrary input order, for one) the pass detects
 //   it and the step is redone in full: the arena range is sorted in place
 //   with a block odd-even merge sort -- sort aligned blocks of BLOCK sheets
@@ -41,7 +41,10 @@ namespace wdy {
 
 namespace {
 
-using BSH = Shape<1024, 4096>;
+// 512 threads, windows of 2048 sheets: 70 KB of LDS, two workgroups per
+// CU, so that one's barriers overlap the other's work (one 1024-thread
+// workgroup with 4096-sheet windows per CU measured 23% slower)
+using BSH = Shape<512, 2048>;
 constexpr int BWT = BSH::WT, BLOCK = BSH::CAP, BPER = BSH::PER;
 constexpr int BNWAVE = BSH::NWAVE;
 
