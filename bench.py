@@ -13,7 +13,8 @@ timed region; the timed region is K output intervals of device work bracketed
 by stream synchronisations.  ``value`` is N * nleap * K / seconds
 (FP64 sheet-steps/s).  The end-to-end rate through the generator loop (each
 output de-sorted and copied to page-locked host arrays while the next
-interval runs) is reported beside it as ``e2e_value``.
+interval runs) is reported beside it as ``e2e_value``, in steady state: timed
+from the first overlapped output on.
 ``--workload unequal27`` measures the large unequal-mass disk instead (N =
 2^27, lognormal(0, 0.5) masses, +0.3 bulk velocity on x > 0, dt = 0.005,
 nleap = 10) -- not the headline line.
