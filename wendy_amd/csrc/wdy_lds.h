@@ -1,7 +1,7 @@
 // LDS building blocks shared by the warm-start step and the batched
 // ensemble kernel: block scans, the (position, sheet index) order, and the
-// in-LDS bucket sort (one counting pass over position bins plus insertion
-// fix-ups of the bins).
+// in-LDS bucket sort (one counting pass over position bins, then each sheet
+// ranks itself within its bin).
 #pragma once
 
 #include "wdy_internal.h"
