@@ -1,6 +1,10 @@
 """The larger workloads at their full sizes, through properties that do not
 need the oracle to finish at that size:
 
+* N = 2^24 equal masses (the single-GPU workload): energy conserved over
+  2000 steps, and the device's energy equal to the oracle's of the same
+  state.
+
 * N = 2^27 unequal masses (lognormal(0, 0.5) normalised to 1, +0.3 bulk
   velocity on x > 0, dt = 0.005, nleap = 10): the warm-start path (bucket
   sort, fixed-point mass prefix) and the radix path (8-pass radix sort, FP64
@@ -19,6 +23,34 @@ from conftest import bits_equal, rel_inf
 from oracle import nbody_oracle as O
 
 pytestmark = pytest.mark.gpu
+
+
+def test_headline_workload_conserves_energy_over_2000_steps(gpu_lib):
+    """N = 2^24, equal masses, omega = 0.1, h = 0.005 / 20: 100 output
+    intervals (2000 leapfrog steps, all but the first interval on the
+    warm-start kernel, none redone).  The total energy, evaluated on the
+    device, stays within 1e-9 relative of its start (measured: ~5e-12; a
+    wrong rank or sign in the force drifts by orders of magnitude more), and
+    the device's energy of the final state equals the oracle's host
+    evaluation of the read-back state to 1e-12."""
+    n = 1 << 24
+    print('energy 2^24: generating', flush=True)
+    x, v, m = O.sech2_disk(n, seed=0)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.1 ** 2.)
+    try:
+        e0 = sim.energy(1., 0.1)
+        for k in range(100):
+            sim.advance(0.005 / 20, 20)
+            if k % 25 == 24:
+                e = sim.energy(1., 0.1)
+                print('energy 2^24: output', k + 1, e / e0 - 1., flush=True)
+                assert abs(e / e0 - 1.) < 1e-9, (k + 1, e / e0 - 1.)
+        assert sim.path_stats() == (1980, 20, 0)
+        xo, vo = numpy.empty(n), numpy.empty(n)
+        sim.read(xo, vo)
+        assert abs(O.energy(xo, vo, m, omega=0.1) / e - 1.) < 1e-12
+    finally:
+        sim.close()
 
 
 def test_large_unequal_workload_paths_agree(gpu_lib):
