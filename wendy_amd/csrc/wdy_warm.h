@@ -25,7 +25,7 @@ constexpr int WARM_NSHAPES = 10;
 // default tile shape (measured at N = 2^24 with the pipelined kernel, equal
 // and unequal masses): 256 threads per workgroup, buckets of up to 1536
 // sheets, target 1229, window 6; four workgroups per CU with equal masses
-// (38 KB of LDS, 128 VGPRs), three with unequal ones.  When sheets cross
+// (38 KB of LDS, at most 128 VGPRs), three with unequal ones.  When sheets cross
 // many buckets per step the window fitting moves to 4096-sheet buckets.
 constexpr int WARM_SHAPE_DEFAULT = 8;
 constexpr int WARM_WINDOW_DEFAULT = 6;
