@@ -26,8 +26,10 @@ constexpr int WARM_NSHAPES = 10;
 // and unequal masses): 256 threads per workgroup, buckets of up to 1536
 // sheets, target 1229, window 6; four workgroups per CU with equal masses
 // (38 KB of LDS, at most 128 VGPRs), three with unequal ones.  When sheets cross
-// many buckets per step the window fitting moves to 4096-sheet buckets.
+// many buckets per step the window fitting moves to 2048-sheet buckets
+// (WARM_SHAPE_WIDE: 256 threads, 1024 bins), then to 4096-sheet ones.
 constexpr int WARM_SHAPE_DEFAULT = 8;
+constexpr int WARM_SHAPE_WIDE = 7;
 constexpr int WARM_WINDOW_DEFAULT = 6;
 const WarmShape &warm_shape(int v);
 inline u32 warm_default_target(int cap) {
