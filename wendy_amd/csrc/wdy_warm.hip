@@ -1380,13 +1380,21 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     if (kind & WARM_ERR_CAPACITY) T = T * 15 / 16 < WARM_T_MIN ? WARM_T_MIN : T * 15 / 16;
     if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) {
       // sheets cross more buckets per step than the window holds: widen it,
-      // and once it is at its limit move to the largest buckets
+      // and once it is at its limit move to larger buckets (2048 sheets,
+      // then 4096), as the window fitting does
       const u32 wmax = (u32)warm_shape(shape).wmax();
       if (W * 2 + 1 <= wmax) {
         W = W * 2 + 1;
+      } else if (warm_shape(shape).cap < warm_shape(WARM_SHAPE_WIDE).cap) {
+        shape = WARM_SHAPE_WIDE;
+        T = 0;
+        const u32 wm = (u32)warm_shape(shape).wmax();
+        W = W < wm ? W : wm;
       } else if (warm_shape(shape).cap < warm_shape(0).cap) {
         shape = 0;
         T = 0;
+        const u32 wm = (u32)warm_shape(shape).wmax();
+        W = W < wm ? W : wm;
       } else {
         W = wmax;
       }
