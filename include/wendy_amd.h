@@ -172,6 +172,13 @@ int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
 int wdy_warm_failures(wdy_sim *sim, int64_t *capacity, int64_t *overflow,
                       int64_t *stall);
 
+/* The warm path's current tile shape (see wdy_tune), bucket target and
+ * window, as chosen by the defaults, the window fitting at the first
+ * interval, failure adaptation or wdy_tune; shape -1 when the sim has no
+ * warm path (small path, WDY_NO_WARM, or disabled after a stall).  No
+ * reference counterpart. */
+int wdy_warm_config(wdy_sim *sim, int *shape, int *bucket_target, int *window);
+
 /* Per-kernel-class device time, for benchmarks.  wdy_set_timing(sim, 1)
  * brackets every subsequent launch with HIP events on the sim's stream;
  * wdy_kernel_time returns the summed milliseconds and launch count of one

@@ -214,6 +214,35 @@ def test_first_warm_interval_after_fitting_succeeds(gpu_lib):
         sim.close()
 
 
+@pytest.mark.parametrize('h,shape', [(0.001, 8), (0.06, 8), (0.15, 7), (0.4, 0)])
+def test_window_fitting_ladder(gpu_lib, h, shape):
+    """The window fitted at the first interval from the sampled displacement
+    (N = 2^18): the default 1536-sheet buckets while the window stays within
+    48 buckets either side, then 2048-sheet buckets up to 60, then 4096-sheet
+    ones; in every case both intervals equal the oracle bit for bit (equal
+    masses 2^-18)."""
+    print('fitting ladder h', h, flush=True)
+    x, v, m = O.sech2_disk(1 << 18, seed=3)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large')
+    st = O.ApproxState(x, v, m, 2 * h, 2, presorted_order=True)
+    xo, vo = numpy.empty(len(x)), numpy.empty(len(x))
+    try:
+        sim.advance(h, 2)
+        sim.read(xo, vo)
+        sh, T, W = sim.warm_config()
+        assert sh == shape, (h, sh, T, W)
+        assert 6 <= W <= {8: 48, 7: 60, 0: 64}[shape], (h, sh, T, W)
+        wx, wv = st.onestep()
+        assert bits_equal(xo, wx) and bits_equal(vo, wv)
+        sim.advance(h, 2)   # warm, or redone on the radix path
+        sim.read(xo, vo)
+        wx, wv = st.onestep()
+        assert bits_equal(xo, wx) and bits_equal(vo, wv)
+        assert sum(sim.path_stats()[:2]) == 4  # warm + radix steps
+    finally:
+        sim.close()
+
+
 def test_autotune_picks_a_shape_and_keeps_results(gpu_lib):
     """Sim.autotune times candidate tile shapes on a copy of the state; the
     simulation itself continues unchanged (its trajectory still matches the

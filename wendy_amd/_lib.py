@@ -36,7 +36,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
-            'wdy_set_ext_force', 'wdy_time', 'wdy_warm_failures',
+            'wdy_set_ext_force', 'wdy_time', 'wdy_warm_failures', 'wdy_warm_config',
             'wdy_read_and_advance', 'wdy_host_alloc', 'wdy_host_free',
             'wdy_exact_create', 'wdy_exact_advance', 'wdy_exact_reset',
             'wdy_exact_read', 'wdy_exact_destroy',
@@ -134,6 +134,8 @@ def _declare(lib):
     lib.wdy_host_free.argtypes = [S]
     lib.wdy_warm_failures.restype = ctypes.c_int
     lib.wdy_warm_failures.argtypes = [S] + [ctypes.POINTER(ctypes.c_int64)] * 3
+    lib.wdy_warm_config.restype = ctypes.c_int
+    lib.wdy_warm_config.argtypes = [S] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.wdy_set_ext_force.restype = ctypes.c_int
     lib.wdy_set_ext_force.argtypes = [S, ctypes.c_int, _F64, ctypes.c_double]
     lib.wdy_time.restype = ctypes.c_int

@@ -531,6 +531,17 @@ int wdy_warm_failures(wdy_sim *sim, int64_t *capacity, int64_t *overflow,
   return WDY_OK;
 }
 
+int wdy_warm_config(wdy_sim *sim, int *shape, int *bucket_target, int *window) {
+  if (int rc = check_sim(sim, "wdy_warm_config")) return rc;
+  int sh = -1;
+  u32 T = 0, W = 0;
+  if (!warm_config(sim, &sh, &T, &W)) sh = -1;
+  if (shape) *shape = sh;
+  if (bucket_target) *bucket_target = (int)T;
+  if (window) *window = (int)W;
+  return WDY_OK;
+}
+
 int wdy_set_timing(wdy_sim *sim, int on) {
   if (int rc = check_sim(sim, "wdy_set_timing")) return rc;
   sim->timing = on != 0;

@@ -235,6 +235,8 @@ int warm_fit(wdy_sim *s, double h);
 void warm_mark_tuned(wdy_sim *s);
 int warm_interval(wdy_sim *s, double h, int nleap, bool *failed);
 int warm_current_shape(wdy_sim *s);
+// the warm path's tile shape, bucket target and window; false without one
+bool warm_config(wdy_sim *s, int *shape, u32 *T, u32 *W);
 int launch_energy_arrays(int64_t n, const double *x, const double *v,
                          const double *m, double twopiG, double omega2,
                          double *e_out);

@@ -1315,6 +1315,14 @@ int warm_current_shape(wdy_sim *s) {
   return s->warm ? s->warm->shape : WARM_SHAPE_DEFAULT;
 }
 
+bool warm_config(wdy_sim *s, int *shape, u32 *T, u32 *W) {
+  if (!s->warm) return false;
+  *shape = s->warm->shape;
+  *T = s->warm->T;
+  *W = s->warm->W;
+  return true;
+}
+
 bool warm_ready_for(wdy_sim *s, double h) {
   return s->warm && s->warm->ready && s->warm->hp == h / 2.;
 }

@@ -151,6 +151,13 @@ class Sim:
         self.tune(0, best[1], best[0])
         return times
 
+    def warm_config(self):
+        """(tile shape, bucket target, window) of the warm path; shape -1
+        without one (see wdy_warm_config)."""
+        c = [ctypes.c_int(0) for _ in range(3)]
+        _lib.check(_lib.lib().wdy_warm_config(self._h, *[ctypes.byref(x) for x in c]))
+        return tuple(x.value for x in c)
+
     def warm_failures(self):
         """(capacity, overflow, stall): warm intervals redone, by cause."""
         c = [ctypes.c_int64(0) for _ in range(3)]
