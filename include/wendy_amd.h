@@ -77,7 +77,14 @@ int wdy_create(int64_t n, const double *x, const double *v, const double *m,
 
 /* Enqueue one output interval: half drift, nleap x (sort, force, kick,
  * drift), last drift halved -- the body of _wendy_nbody_approx_onestep
- * (wendy/wendy.c:394-411) with h = dt/nleap.  Asynchronous. */
+ * (wendy/wendy.c:394-411) with h = dt/nleap.  Asynchronous: it returns
+ * once the interval is enqueued.  The device's verdict on a warm-start
+ * interval (a bucket that outgrew its capacity, too many far movers) is
+ * checked by the next call on the handle that reads or extends the state
+ * (wdy_advance, wdy_read, wdy_read_and_advance, wdy_energy, wdy_tune, the
+ * ext-force setters and the statistics), which then redoes the interval from
+ * its untouched start on the radix path before doing its own work; so a
+ * result never depends on whether a redo happened. */
 int wdy_advance(wdy_sim *sim, double h, int nleap);
 
 /* De-sort the state into original sheet order and copy it to the host:
@@ -141,10 +148,11 @@ int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window);
 /* External force evaluated on the device, the counterpart of nbody's
  * ext_force callback (wendy/wendy.py:389-420; the C side adds it at
  * wendy/wendy.c:362-370 and advances *t0 by h after every force evaluation,
- * :401-402,407-408).  A callback into host code would break device
- * residency, so the force is one of a menu of forms, evaluated at the sorted
- * position x and the step's force time t (t0 at the first force of an
- * output interval, then +h per step, the reference's clock exactly):
+ * :401-402,407-408).  Evaluated inside the step kernels, so the state never
+ * leaves the device, for a menu of forms (any other function goes through
+ * wdy_set_ext_force_host), at the sorted position x and the step's force
+ * time t (t0 at the first force of an output interval, then +h per step, the
+ * reference's clock exactly):
  *   WDY_EXT_HARMONIC    -p0 * x                              (p0 = omega**2)
  *   WDY_EXT_ISOTHERMAL  (p0 * tanh((0.5 * x) / p1)) / p1     (amp, zh)
  *   WDY_EXT_PULSE       p0 * exp(-0.5 * ((t - p1) / p2)**2)  (amp, tc, tau)
@@ -156,6 +164,24 @@ int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window);
 #define WDY_EXT_ISOTHERMAL 2
 #define WDY_EXT_PULSE 3
 int wdy_set_ext_force(wdy_sim *sim, int kind, const double *params, double t0);
+/* External force given as a host function with the reference's callback
+ * signature, double f(int N, double *x, double t, double *a): the
+ * counterpart of an arbitrary ext_force callable (wendy/wendy.py:389-420,
+ * called at wendy/wendy.c:362-370).  Before every leapfrog step the
+ * positions of the force evaluation are copied to the host and the function
+ * is called the way the reference calls it -- above 10 sheets once with the
+ * de-sorted positions x[N] (it writes a[N]); for up to 10 sheets once per
+ * sheet in sorted order with N = 1, a = NULL, the return value being that
+ * sheet's acceleration -- then the values are uploaded and added to the
+ * self-gravity (a = E + F).  t0 sets the clock, which advances by h after
+ * every evaluation.  Each step costs a device-host round trip; the state
+ * stays on the device.  The simulation must be on the large path
+ * (WDY_FORCE_LARGE, or n above the small-path capacity): otherwise
+ * WDY_EUNSUPPORTED.  NULL removes the force.  Replaces any force set by
+ * wdy_set_ext_force, and the reverse. */
+int wdy_set_ext_force_host(wdy_sim *sim,
+                           double (*force)(int, double *, double, double *),
+                           double t0);
 /* The simulation clock t0 (advanced only while an external force is set). */
 int wdy_time(wdy_sim *sim, double *t_out);
 
@@ -296,6 +322,39 @@ int wdy_exact_reset(wdy_exact *ex);
 int wdy_exact_read(wdy_exact *ex, double *x_out, double *v_out);
 int wdy_exact_destroy(wdy_exact *ex);
 
+/* ---- reference-signature exact-solver entry points -------------------------
+ * Same names, arguments and meaning as the reference's, so that its loader
+ * (wendy/wendy.py:40-72,94-101 binds all four at import) finds them in this
+ * library.  The caller keeps the solver state between calls exactly as the
+ * reference's set-up (wendy/wendy.py:243-279) lays it out: x, v, a per sheet
+ * (a: force, or with omega the equilibrium offset), m (already x twopiG, and
+ * / omega**2 with omega), sindx (the sheet at each position rank), tcoll[N-1]
+ * (crossing times of neighbouring ranks), *cindx / *next_tcoll (the earliest
+ * crossing).  One call runs one output interval dt on the host CPU with the
+ * same event queue as wdy_exact_*: x, v, a, sindx, tcoll, *cindx and
+ * *next_tcoll are updated in place, with times shifted to the interval's
+ * end; *ncoll receives the crossings processed, *time_elapsed the CPU time of
+ * the crossing loop, and *err is set to -2 when maxcoll crossings were
+ * processed (left untouched otherwise), as in the reference.  Crossing times
+ * that are exactly equal are taken in pair-index order.
+ *   _wendy_nbody_onestep       replaces wendy/wendy.c:53-165
+ *   _wendy_nbody_harm_onestep  replaces wendy/wendy.c:167-315 (omega > 0;
+ *                              otherwise *err = WDY_EINVAL, nothing moves)
+ *   _solve_coll_quad           replaces wendy/wendy.c:33-38: the root
+ *                              0.5 * (r + sqrt(r*r - 4 c0/c2)), r = -c1/c2
+ *   _solve_coll_harm           replaces wendy/wendy.c:39-52 */
+void _wendy_nbody_onestep(int N, double *x, double *v, double *a, double *m,
+                          int *sindx, int *cindx, double *next_tcoll,
+                          double *tcoll, double dt, int maxcoll, int *err,
+                          int *ncoll, double *time_elapsed);
+void _wendy_nbody_harm_onestep(int N, double *x, double *v, double *a,
+                               double *m, int *sindx, int *cindx,
+                               double *next_tcoll, double *tcoll, double dt,
+                               int maxcoll, int *err, int *ncoll,
+                               double *time_elapsed, double omega);
+double _solve_coll_quad(double c0, double c1, double c2);
+double _solve_coll_harm(double c0, double c1, double c2, double omega);
+
 /* ---- reference-signature compatibility entry point ----------------------
  * Same name, arguments and meaning as the reference's
  * _wendy_nbody_approx_onestep (wendy/wendy.c:385-393, wendy/wendy.h:29-34),
@@ -304,11 +363,14 @@ int wdy_exact_destroy(wdy_exact *ex);
  * nleap leapfrog steps on the GPU, and writes back xi (sorted), x
  * (de-sorted), v, a (last force, original order) and cumulmass (last mass
  * prefix, sorted order) and *time_elapsed (device seconds).  *t0 is not
- * touched (it only advances with ext_force).  Unlike the reference, which
- * ignores both, a non-NULL ext_force sets *err = WDY_EUNSUPPORTED and an
- * unknown sort_type (not 0..4) sets *err = WDY_EINVAL, without computing;
- * every sort_type maps to the device sort.  Other failures set *err to the
- * WDY_E* code. */
+ * touched unless ext_force is given: then the function is called before
+ * every step as wdy_set_ext_force_host describes (the reference's own
+ * calling convention) and *t0 advances by dt per step, as in the reference.
+ * Every sort_type maps to the device sort; unlike the reference, which
+ * silently skips the sort, an unknown sort_type (not 0..4) sets
+ * *err = WDY_EINVAL without computing.  Other failures set *err to the
+ * WDY_E* code; after any failure x, v and xi[].val hold NaN and a message
+ * goes to stderr (the reference's caller never reads err). */
 struct array_w_index {
   int idx;
   double val;

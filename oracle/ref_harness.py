@@ -40,19 +40,39 @@ def available(variant=''):
 
 _libs = {}
 
+_I32 = ndpointer(dtype=numpy.int32, flags=('C_CONTIGUOUS', 'WRITEABLE'))
+
+
+def bind(lib):
+    """Declares on ``lib`` the five entry points the reference's loader binds
+    at import, with its argument and return types (wendy/wendy.py:38-101),
+    and returns ``lib``.  Any library that exports them can be bound."""
+    exact = [ctypes.c_int, _F64, _F64, _F64, _F64, _I32,
+             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+             _F64, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    lib._wendy_nbody_onestep.restype = None
+    lib._wendy_nbody_onestep.argtypes = exact
+    lib._wendy_nbody_harm_onestep.restype = None
+    lib._wendy_nbody_harm_onestep.argtypes = exact + [ctypes.c_double]
+    f = lib._wendy_nbody_approx_onestep
+    f.restype = None
+    f.argtypes = [ctypes.c_int, _XI, _F64, _F64, _F64, _F64,
+                  ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                  ctypes.c_void_p, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_int),
+                  ctypes.POINTER(ctypes.c_double), _F64]
+    lib._solve_coll_quad.restype = ctypes.c_double
+    lib._solve_coll_quad.argtypes = [ctypes.c_double] * 3
+    lib._solve_coll_harm.restype = ctypes.c_double
+    lib._solve_coll_harm.argtypes = [ctypes.c_double] * 4
+    return lib
+
 
 def load(variant=''):
     if variant not in _libs:
-        lib = ctypes.CDLL(lib_path(variant))
-        f = lib._wendy_nbody_approx_onestep
-        f.restype = None
-        f.argtypes = [ctypes.c_int, _XI, _F64, _F64, _F64, _F64,
-                      ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                      ctypes.POINTER(ctypes.c_double), ctypes.c_double,
-                      ctypes.c_void_p, ctypes.c_int,
-                      ctypes.POINTER(ctypes.c_int),
-                      ctypes.POINTER(ctypes.c_double), _F64]
-        _libs[variant] = lib
+        _libs[variant] = bind(ctypes.CDLL(lib_path(variant)))
     return _libs[variant]
 
 
@@ -65,8 +85,8 @@ class RefApprox:
     """
 
     def __init__(self, x, v, m, dt, nleap, omega=None, twopiG=1.,
-                 sort='quick', variant='', ext_force=None, t0=0.):
-        self.lib = load(variant)
+                 sort='quick', variant='', ext_force=None, t0=0., lib=None):
+        self.lib = load(variant) if lib is None else lib
         self.ext_c = None
         if ext_force is not None:
             proto = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int,
@@ -125,22 +145,8 @@ class RefExact:
     """
 
     def __init__(self, x, v, m, dt, omega=None, twopiG=1., maxcoll=100000,
-                 variant=''):
-        lib = load(variant)
-        self.lib = lib
-        i32 = ndpointer(dtype=numpy.int32, flags=('C_CONTIGUOUS', 'WRITEABLE'))
-        base = [ctypes.c_int, _F64, _F64, _F64, _F64, i32,
-                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
-                _F64, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
-        lib._wendy_nbody_onestep.restype = None
-        lib._wendy_nbody_onestep.argtypes = base
-        lib._wendy_nbody_harm_onestep.restype = None
-        lib._wendy_nbody_harm_onestep.argtypes = base + [ctypes.c_double]
-        lib._solve_coll_quad.restype = ctypes.c_double
-        lib._solve_coll_quad.argtypes = [ctypes.c_double] * 3
-        lib._solve_coll_harm.restype = ctypes.c_double
-        lib._solve_coll_harm.argtypes = [ctypes.c_double] * 4
+                 variant='', lib=None):
+        self.lib = load(variant) if lib is None else lib
         self.omega = omega
         self.dt = dt
         self.maxcoll = maxcoll

@@ -20,7 +20,8 @@ def header_functions():
     text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
     names = re.findall(r'\b(\w+)\s*\([^;{]*\)\s*;', text)
     return sorted(set(n for n in names
-                      if n.startswith('wdy_') or n.startswith('_wendy')))
+                      if n.startswith('wdy_') or n.startswith('_wendy')
+                      or n.startswith('_solve_coll')))
 
 
 def have_gpu():
@@ -91,43 +92,54 @@ def test_unknown_sort_raises_keyerror_lazily():
         next(g)
 
 
-def test_python_callable_ext_force_not_provided():
-    """Only the device-evaluated forms of wendy_amd.forces are accepted."""
-    g = wendy_amd.nbody(numpy.zeros(2), numpy.zeros(2), numpy.ones(2), 0.1,
-                        approx=True, nleap=2, ext_force=lambda x, t: 0.)
-    with pytest.raises(NotImplementedError):
-        next(g)
+def test_ext_force_callback_follows_reference_convention():
+    """A Python ext_force is handed to the library as the reference's C
+    callback (wendy/wendy.py:406-418): one sheet -> the return value; N
+    sheets -> the values stored into a[:N], return 0."""
+    calls = []
+
+    def f(x, t):
+        calls.append(t)
+        return -2. * x + t
+
+    cb = _lib.ext_force_callback(f)
+    one = ctypes.c_double(1.5)
+    assert cb(1, ctypes.pointer(one), 0.25, None) == -2.75
+    x = numpy.array([0., 1., -3.])
+    a = numpy.zeros(3)
+    r = cb(3, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0.5,
+           a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert r == 0.
+    assert numpy.array_equal(a, -2. * x + 0.5)
+    assert calls == [0.25, 0.5]
 
 
-def test_compat_symbol_rejects_ext_force_and_bad_sort():
-    """The reference-signature entry point refuses what it cannot do, without
-    touching the device."""
+def test_compat_symbol_rejects_bad_sort_and_poisons_state():
+    """An unknown sort_type (the reference silently skips its sort) is
+    refused before the device is touched; err is set and, since the
+    reference's caller never reads err, the handed-back x, v and xi[].val
+    are NaN rather than an un-advanced state."""
     lib = ctypes.CDLL(_lib.LIB_PATH)
     f = lib._wendy_nbody_approx_onestep
     f.restype = None
     xi = numpy.zeros(2, dtype=[('idx', '<i4'), ('pad', '<i4'), ('val', '<f8')])
+    xi['idx'] = [0, 1]
+    x = numpy.array([0.5, -0.5])
+    v = numpy.array([1., 2.])
     z = numpy.zeros(2)
+    m = numpy.ones(2)
     err = ctypes.c_int(0)
     te = ctypes.c_double(0)
     t0 = ctypes.c_double(0)
-    cb = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int,
-                          ctypes.POINTER(ctypes.c_double), ctypes.c_double,
-                          ctypes.POINTER(ctypes.c_double))(lambda *a: 0.)
-    args = lambda ext, st: (ctypes.c_int(2), xi.ctypes.data_as(ctypes.c_void_p),
-                            z.ctypes.data_as(ctypes.c_void_p),
-                            z.ctypes.data_as(ctypes.c_void_p),
-                            z.ctypes.data_as(ctypes.c_void_p),
-                            z.ctypes.data_as(ctypes.c_void_p),
-                            ctypes.c_double(1.), ctypes.c_double(0.1),
-                            ctypes.c_int(2), ctypes.byref(t0),
-                            ctypes.c_double(-1.), ext, ctypes.c_int(st),
-                            ctypes.byref(err), ctypes.byref(te),
-                            z.ctypes.data_as(ctypes.c_void_p))
-    f(*args(cb, 0))
-    assert err.value == _lib.WDY_EUNSUPPORTED
-    err.value = 0
-    f(*args(None, 7))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    f(ctypes.c_int(2), p(xi), p(x), p(v), p(m), p(z), ctypes.c_double(2.),
+      ctypes.c_double(0.1), ctypes.c_int(2), ctypes.byref(t0),
+      ctypes.c_double(-1.), None, ctypes.c_int(7), ctypes.byref(err),
+      ctypes.byref(te), p(z))
     assert err.value == _lib.WDY_EINVAL
+    assert numpy.all(numpy.isnan(x)) and numpy.all(numpy.isnan(v))
+    assert numpy.all(numpy.isnan(xi['val']))
+    assert numpy.array_equal(m, numpy.ones(2))
 
 
 def test_batch_create_rejects_bad_arguments_before_touching_device():

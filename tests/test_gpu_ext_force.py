@@ -4,8 +4,11 @@ whose external-force step and clock equal the compiled reference bit for bit
 (tests/test_oracle.py).  Bars: bitwise for the harmonic form with
 power-of-two equal masses (E = -p0 x and F are exact mirrors), else
 ||.||_inf relative 1e-10 (the device's tanh and exp may differ from numpy's by
-an ulp).  Plus the reference's own tests/test_approx_external.py with the
-lambda replaced by the equivalent device form."""
+an ulp).  Arbitrary Python callables are called on the host before every
+step (the reference's own callback convention) and are checked against the
+same oracle.  Plus the reference's own tests/test_approx_external.py: its
+long runs with the device form of its harmonic lambda, and its lambda and
+force class themselves on the host-callback path over fewer outputs."""
 import math
 
 import numpy
@@ -14,7 +17,7 @@ import pytest
 import wendy_amd
 from conftest import bits_equal, rel_inf
 from oracle import nbody_oracle as O
-from wendy_amd import forces
+from wendy_amd import _lib, forces
 
 pytestmark = pytest.mark.gpu
 
@@ -100,12 +103,20 @@ def test_pulse_momentum_on_device(gpu_lib):
     assert abs(dp - math.fsum(m) * h * math.fsum(f(0., tk) for tk in ts)) <= 1e-12
 
 
-def test_nbody_accepts_menu_and_rejects_callables(gpu_lib):
+def test_nbody_accepts_menu_and_callables(gpu_lib):
+    """A device form and an arbitrary Python callable (called on the host
+    before every step, as the reference calls its ext_force) both match the
+    oracle, whose callback path equals the compiled reference's bit for
+    bit; with power-of-two equal masses the callable's run is bitwise."""
     x, v, m = O.sech2_disk(64, seed=2)
-    g = wendy_amd.nbody(x, v, m, 0.05, approx=True, nleap=10,
-                        ext_force=lambda x, t: -x)
-    with pytest.raises(NotImplementedError):
-        next(g)
+    f = lambda y, t: -1.3 * y + 0.2 * numpy.cos(t)
+    g = wendy_amd.nbody(x, v, m, 0.05, approx=True, nleap=10, t0=0.4,
+                        ext_force=f)
+    st = O.ApproxState(x, v, m, 0.05, 10, ext_force=f, t0=0.4)
+    for _ in range(3):
+        gx, gv = next(g)
+        wx, wv = st.onestep()
+        assert bits_equal(gx, wx) and bits_equal(gv, wv)
     g = wendy_amd.nbody(x, v, m, 0.05, approx=True, nleap=10, t0=0.5,
                         ext_force=forces.Pulse(1., 0.5, 0.1))
     st = O.ApproxState(x, v, m, 0.05, 10, ext_force=forces.Pulse(1., 0.5, 0.1), t0=0.5)
@@ -113,6 +124,50 @@ def test_nbody_accepts_menu_and_rejects_callables(gpu_lib):
         gx, gv = next(g)
         wx, wv = st.onestep()
         assert rel_inf(gx, wx) <= TOL and rel_inf(gv, wv) <= TOL
+
+
+@pytest.mark.parametrize('n,mass', [(3, 'three'), (101, 'lognormal'),
+                                    (20000, 'lognormal')])
+def test_host_callable_against_oracle(gpu_lib, n, mass):
+    """Host-evaluated forces on the large path, sheet by sheet (n <= 10)
+    and on the whole array, against the oracle to 1e-10 (the mass prefix
+    is reassociated); the clock bit for bit."""
+    print('ext host', n, mass, flush=True)
+    if mass == 'three':
+        x, v, m = (numpy.array([-1.1, 0.1, 1.3]), numpy.array([3., 2., -5.]),
+                   numpy.array([1., 2., 3.]))
+    else:
+        x, v, m = O.sech2_disk(n, seed=n, mass=('lognormal', 0.2))
+
+    class Tide:  # a callable object, as in the reference's tests
+        def __call__(self, y, t):
+            return 0.4 * numpy.tanh(y) * numpy.exp(-t)
+
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large', warm=False)
+    f = Tide()
+    sim.set_ext_force(f, 0.1)
+    st = O.ApproxState(x, v, m, 0.01 * 7, 7, ext_force=f, t0=0.1)
+    xo = numpy.empty(len(x))
+    vo = numpy.empty(len(x))
+    try:
+        for k in range(3):
+            sim.advance(0.01, 7)
+            sim.read(xo, vo)
+            wx, wv = st.onestep()
+            assert rel_inf(xo, wx) <= TOL and rel_inf(vo, wv) <= TOL, k
+            assert sim.time() == st.t
+    finally:
+        sim.close()
+
+
+def test_host_callable_refused_on_small_path(gpu_lib):
+    x, v, m = O.sech2_disk(64, seed=2)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='small')
+    try:
+        with pytest.raises(_lib.WendyError):
+            sim.set_ext_force(lambda y, t: -y, 0.)
+    finally:
+        sim.close()
 
 
 # ---- the reference's tests/test_approx_external.py ---------------------------
@@ -154,17 +209,39 @@ def test_reference_energy_conservation_sech2disk(gpu_lib):
         assert abs(wendy_amd.energy(tx, tv, m, omega=omega) - E) / E < 1e-6
 
 
-def test_reference_external_equals_builtin_omega(gpu_lib):
-    """tests/test_approx_external.py:99-113 compares against the exact solver
-    with omega; here the same run with the built-in omega term (the exact
-    solver is a CPU check not provided by this library) at the same 1e-5."""
+def test_reference_external_against_exact(gpu_lib):
+    """tests/test_approx_external.py:99-113: the approximate integration with
+    the harmonic force given as ext_force agrees with the exact solver run
+    with omega to 1e-5 over 100 outputs (nleap = 2000)."""
     x, v, m = _disk101(numpy.random.RandomState(3))
     omega = 1.1
     g = wendy_amd.nbody(x, v, m, 0.05, approx=True, nleap=2000,
                         ext_force=forces.Harmonic(omega))
-    ge = wendy_amd.nbody(x, v, m, 0.05, approx=True, nleap=2000, omega=omega)
+    ge = wendy_amd.nbody(x, v, m, 0.05, omega=omega)
     for _ in range(100):
         tx, tv = next(g)
         txe, tve = next(ge)
         assert numpy.all(numpy.fabs(tx - txe) < 1e-5)
         assert numpy.all(numpy.fabs(tv - tve) < 1e-5)
+
+
+@pytest.mark.parametrize('wclass', [False, True])
+def test_reference_external_lambda_energy_sech2disk(gpu_lib, wclass):
+    """tests/test_approx_external.py:45-97 with the reference's own lambda
+    and force class (host-evaluated: one device-host round trip per step),
+    over 10 of its 100 outputs."""
+    x, v, m = _disk101(numpy.random.RandomState(2))
+    omega = 1.1
+
+    class Eforce(object):
+        def __init__(self, omega):
+            self._omega2 = omega ** 2.
+
+        def __call__(self, x, t):
+            return -self._omega2 * x
+    f = Eforce(omega) if wclass else (lambda x, t: -omega ** 2. * x)
+    g = wendy_amd.nbody(x, v, m, 0.05, ext_force=f, approx=True, nleap=1000)
+    E = wendy_amd.energy(x, v, m, omega=omega)
+    for _ in range(10):
+        tx, tv = next(g)
+        assert abs(wendy_amd.energy(tx, tv, m, omega=omega) - E) / E < 1e-6

@@ -308,3 +308,68 @@ def test_warm_cold_collapse(gpu_lib):
         assert warm + cold == 80
     finally:
         sim.close()
+
+
+def test_advance_returns_before_the_interval_finishes(gpu_lib):
+    """wdy_advance only enqueues a warm interval: the host call returns in a
+    fraction of the interval's device time (the error word is checked by the
+    next call on the handle)."""
+    import time
+    n = 1 << 23
+    x, v, m = make(n, 12)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.01, path='large')
+    try:
+        h, nleap = 0.005 / 40, 40
+        for _ in range(2):                  # cold interval, then a warm one
+            sim.advance(h, nleap)
+            sim.read()
+        t0 = time.perf_counter()
+        sim.advance(h, nleap)
+        host = time.perf_counter() - t0
+        dev = sim.read()
+        warm, cold, redo = sim.path_stats()
+        print('advance host %.3f ms, interval %.3f ms' % (host * 1e3, dev * 1e3),
+              flush=True)
+        assert redo == 0 and warm == 2 * nleap
+        assert dev > 2e-3 and host < 0.5 * dev
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize('reader', ['advance', 'read_and_advance'])
+def test_warm_failure_found_by_the_next_call(gpu_lib, reader):
+    """Intervals whose warm steps fail on the device (window 0 and tiny
+    buckets: the overflow list runs over) are enqueued without waiting; the
+    next call on the handle -- another advance, or the generator's
+    read_and_advance -- finds the failure, rewinds and redoes the interval on
+    the radix path before its own work.  Outputs stay the oracle's."""
+    n = 1 << 18
+    x, v, m = make(n, 3)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large')
+    sim.tune(32, 0)
+    dt, nleap = 0.02, 5
+    st = O.ApproxState(x, v, m, dt, nleap)
+    xo = numpy.empty(n)
+    vo = numpy.empty(n)
+    try:
+        sim.advance(dt / nleap, nleap)      # cold: builds the warm layout
+        want = [st.onestep()]
+        if reader == 'advance':
+            for _ in range(3):              # each settles the one before
+                sim.advance(dt / nleap, nleap)
+                want.append(st.onestep())
+            sim.read(xo, vo)
+            check((xo, vo), want[-1], True, 'after 4 outputs')
+        else:
+            for k in range(4):
+                sim.read_and_advance(xo, vo, dt / nleap, nleap)
+                check((xo, vo), want[k], True, 'output %d' % (k + 1))
+                want.append(st.onestep())
+            sim.read(xo, vo)
+            check((xo, vo), want[4], True, 'output 5')
+        warm, cold, redo = sim.path_stats()
+        print('deferred redo', reader, warm, cold, redo, sim.warm_failures(),
+              flush=True)
+        assert redo >= 1
+    finally:
+        sim.close()

@@ -36,7 +36,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
-            'wdy_set_ext_force', 'wdy_time', 'wdy_warm_failures', 'wdy_warm_config',
+            'wdy_set_ext_force', 'wdy_set_ext_force_host', 'wdy_time', 'wdy_warm_failures', 'wdy_warm_config',
             'wdy_read_and_advance', 'wdy_host_alloc', 'wdy_host_free',
             'wdy_exact_create', 'wdy_exact_advance', 'wdy_exact_reset',
             'wdy_exact_read', 'wdy_exact_destroy',
@@ -45,7 +45,9 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_pool_create', 'wdy_pool_add', 'wdy_pool_step', 'wdy_pool_read',
             'wdy_pool_remove', 'wdy_pool_sync', 'wdy_pool_stats',
             'wdy_pool_destroy',
-            '_wendy_nbody_approx_onestep')
+            '_wendy_nbody_approx_onestep', '_wendy_nbody_onestep',
+            '_wendy_nbody_harm_onestep', '_solve_coll_quad',
+            '_solve_coll_harm')
 
 _F64 = ndpointer(dtype=numpy.float64, flags=('C_CONTIGUOUS',))
 _F64W = ndpointer(dtype=numpy.float64, flags=('C_CONTIGUOUS', 'WRITEABLE'))
@@ -138,6 +140,8 @@ def _declare(lib):
     lib.wdy_warm_config.argtypes = [S] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.wdy_set_ext_force.restype = ctypes.c_int
     lib.wdy_set_ext_force.argtypes = [S, ctypes.c_int, _F64, ctypes.c_double]
+    lib.wdy_set_ext_force_host.restype = ctypes.c_int
+    lib.wdy_set_ext_force_host.argtypes = [S, ctypes.c_void_p, ctypes.c_double]
     lib.wdy_time.restype = ctypes.c_int
     lib.wdy_time.argtypes = [S, _c_double_p]
     _I32 = ndpointer(dtype=numpy.int32, flags=('C_CONTIGUOUS',))
@@ -192,6 +196,30 @@ def check(rc):
     if rc == WDY_ENOMEM:
         raise MemoryError(msg)
     raise WendyError(rc, msg)
+
+
+# the reference's callback type for an external force (wendy/wendy.py:406-410)
+EXT_FORCE_CFUNC = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double),
+                                   ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_double))
+
+
+def ext_force_callback(f):
+    """Wraps a Python callable ``f(x, t)`` as the C callback the library
+    calls before each step, with the reference's conventions
+    (wendy/wendy.py:411-418): for one sheet it returns ``f(x, t)``, for N
+    sheets it stores ``f(x, t)`` into a[:N]."""
+    from numpy.ctypeslib import as_array
+
+    def arrays(n, xp, t, ap):
+        if n == 1:
+            return float(f(xp.contents.value, t))
+        xa = as_array(xp, shape=(n,))
+        aa = as_array(ap, shape=(n,))
+        aa[:] = f(xa, t)
+        return 0.
+    return EXT_FORCE_CFUNC(arrays)
 
 
 def f64(a):

@@ -131,6 +131,7 @@ static void free_sim(wdy_sim *s) {
   if (s->red) (void)hipFree(s->red);
   if (s->out_x) (void)hipFree(s->out_x);
   if (s->out_v) (void)hipFree(s->out_v);
+  if (s->ext_tab) (void)hipFree(s->ext_tab);
   radix_free(&s->radix);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -283,53 +284,151 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
   return WDY_OK;
 }
 
+// Inputs of the number of sheets up to which the reference evaluates an
+// external force sheet by sheet in sorted order rather than on the whole
+// de-sorted array (EXTERNAL_SWITCH, wendy/wendy.h:9-11, wendy/wendy.c:362-370).
+constexpr int64_t EXT_SWITCH = 10;
+
+// One output interval with the external force of a host function: the
+// radix-path steps, with the positions of each step's force evaluation
+// copied out, the function called on them as the reference calls its
+// callback (wendy/wendy.c:362-370: the whole de-sorted array above
+// EXT_SWITCH sheets, else one sheet at a time in sorted order), and its
+// values uploaded to the per-sheet table the step kernel adds to the
+// self-gravity (a = E + F).  The clock t0 advances by h after every force
+// evaluation (wendy/wendy.c:401-402,407-408).
+static int advance_host_force(wdy_sim *s, double h, int nleap,
+                              double *acc_last, double *cum_last) {
+  const size_t N = (size_t)s->n;
+  // [0, N): the force table; [N, 2N): positions handed to the function (not
+  // out_x, which an overlapped output copy may still be reading)
+  if (!s->ext_tab) WDY_HIP(hipMalloc(&s->ext_tab, 2 * N * sizeof(double)));
+  double *const xpos = s->ext_tab + N;
+  std::vector<double> hx(N), ha(N);
+  std::vector<u64> hk;
+  std::vector<i32> hi;
+  ExtForce f;
+  f.kind = EXT_TABLE;
+  f.tab = s->ext_tab;
+  const ExtForce saved = s->ext;
+  s->ext = f;
+  int rc = [&]() -> int {
+    if (int r = launch_drift(s, h / 2.)) return r;
+    double t = s->t0;
+    for (int ii = 0; ii < nleap; ++ii) {
+      const bool last = ii == nleap - 1;
+      if ((int64_t)N > EXT_SWITCH) {
+        if (int r = launch_desort(s, xpos, nullptr, nullptr, nullptr, nullptr))
+          return r;
+        WDY_HIP(hipMemcpyAsync(hx.data(), xpos, N * 8, hipMemcpyDeviceToHost,
+                               s->stream));
+        WDY_HIP(hipStreamSynchronize(s->stream));
+        (void)s->host_force((int)N, hx.data(), t, ha.data());
+      } else {
+        hk.resize(N);
+        hi.resize(N);
+        WDY_HIP(hipMemcpyAsync(hk.data(), s->key[s->cur], N * 8,
+                               hipMemcpyDeviceToHost, s->stream));
+        WDY_HIP(hipMemcpyAsync(hi.data(), s->idx[s->cur], N * 4,
+                               hipMemcpyDeviceToHost, s->stream));
+        WDY_HIP(hipStreamSynchronize(s->stream));
+        std::vector<size_t> o(N);
+        for (size_t q = 0; q < N; ++q) o[q] = q;
+        std::sort(o.begin(), o.end(), [&](size_t p, size_t q) {
+          return hk[p] < hk[q] || (hk[p] == hk[q] && hi[p] < hi[q]);
+        });
+        for (size_t q : o) {
+          double xq = decode_pos(hk[q]);
+          ha[(size_t)hi[q]] = s->host_force(1, &xq, t, nullptr);
+        }
+      }
+      WDY_HIP(hipMemcpyAsync(s->ext_tab, ha.data(), N * 8, hipMemcpyHostToDevice,
+                             s->stream));
+      if (int r = launch_leap(s, h, last ? h / 2. : h, last ? acc_last : nullptr,
+                              last ? cum_last : nullptr, nullptr, 0u, 0u, t))
+        return r;
+      t = t + h;
+    }
+    // the host arrays are reused by the next step: wait for the upload
+    WDY_HIP(hipStreamSynchronize(s->stream));
+    s->t0 = t;
+    s->cold_steps += nleap;
+    warm_invalidate(s);  // the warm layout no longer describes the state
+    return WDY_OK;
+  }();
+  s->ext = saved;
+  return rc;
+}
+
+// One output interval on the radix path (a full sort every step), then,
+// when the sim has a warm path, the partition that hands the next interval
+// to it.
+static int cold_interval(wdy_sim *s, double h, int nleap, double *acc_last,
+                         double *cum_last) {
+  // size the warm-start layout for this h before the splitters of the
+  // next partition are written (a resize reallocates their buffers)
+  if (s->warm) {
+    if (int rc = warm_fit(s, h)) return rc;
+  }
+  int rc = launch_drift(s, h / 2.);
+  if (rc) return rc;
+  u32 T = 0, nb = 0;
+  u64 *spl = s->warm ? warm_cold_splitters(s, &T, &nb) : nullptr;
+  double t = s->t0;  // force time of each step (wendy/wendy.c:401-408)
+  for (int ii = 0; ii < nleap; ++ii) {
+    const bool last = ii == nleap - 1;
+    rc = launch_leap(s, h, last ? h / 2. : h, last ? acc_last : nullptr,
+                     last ? cum_last : nullptr, last ? spl : nullptr, T, nb, t);
+    if (rc) return rc;
+    t = t + h;
+  }
+  if (s->ext.kind) s->t0 = t;
+  s->cold_steps += nleap;
+  if (s->warm) {
+    rc = warm_partition_after_cold(s, h / 2.);
+    if (rc) return rc;
+  }
+  return WDY_OK;
+}
+
+// Completes the warm interval in flight, if any: waits for its error word
+// and, when the device reported a failure (a bucket over capacity, too many
+// far movers, a bounded wait exceeded), redoes the interval from its
+// untouched start on the radix path.  Every entry point that reads the
+// state or extends it calls this first.
+static int settle(wdy_sim *s) {
+  if (!s->warm) return WDY_OK;
+  WDY_HIP(hipSetDevice(s->device));
+  bool failed = false;
+  double h = 0.0;
+  int nleap = 0;
+  if (int rc = warm_settle(s, &failed, &h, &nleap)) return rc;
+  if (!failed) return WDY_OK;
+  s->redos += 1;
+  if (int rc = cold_interval(s, h, nleap, nullptr, nullptr)) return rc;
+  WDY_HIP(hipEventRecord(s->ev1, s->stream));
+  return WDY_OK;
+}
+
 static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
                         double *cum_last) {
   WDY_HIP(hipSetDevice(s->device));
+  if (int rc = settle(s)) return rc;
   WDY_HIP(hipEventRecord(s->ev0, s->stream));
-  if (s->small) {
+  if (s->host_force) {
+    if (int rc = advance_host_force(s, h, nleap, acc_last, cum_last)) return rc;
+  } else if (s->small) {
     int rc = launch_small_advance(s, h, nleap, acc_last, cum_last);
     if (rc) return rc;
     if (s->ext.kind)  // the kernel's clock, replayed (wendy/wendy.c:401-408)
       for (int ii = 0; ii < nleap; ++ii) s->t0 = s->t0 + h;
-  } else {
+  } else if (!acc_last && !cum_last && warm_ready_for(s, h)) {
     // warm interval when the state is partitioned for this h and nothing
-    // asks for the last force; on a device-side failure it rewinds and the
-    // interval is redone on the radix path below
-    if (!acc_last && !cum_last && warm_ready_for(s, h)) {
-      bool failed = false;
-      int rc = warm_interval(s, h, nleap, &failed);
-      if (rc) return rc;
-      if (!failed) {
-        s->warm_steps += nleap;
-        WDY_HIP(hipEventRecord(s->ev1, s->stream));
-        return WDY_OK;
-      }
-      s->redos += 1;
-    }
-    // size the warm-start layout for this h before the splitters of the
-    // next partition are written (a resize reallocates their buffers)
-    if (s->warm) {
-      if (int rc = warm_fit(s, h)) return rc;
-    }
-    int rc = launch_drift(s, h / 2.);
-    if (rc) return rc;
-    u32 T = 0, nb = 0;
-    u64 *spl = s->warm ? warm_cold_splitters(s, &T, &nb) : nullptr;
-    double t = s->t0;  // force time of each step (wendy/wendy.c:401-408)
-    for (int ii = 0; ii < nleap; ++ii) {
-      const bool last = ii == nleap - 1;
-      rc = launch_leap(s, h, last ? h / 2. : h, last ? acc_last : nullptr,
-                       last ? cum_last : nullptr, last ? spl : nullptr, T, nb, t);
-      if (rc) return rc;
-      t = t + h;
-    }
-    if (s->ext.kind) s->t0 = t;
-    s->cold_steps += nleap;
-    if (s->warm) {
-      rc = warm_partition_after_cold(s, h / 2.);
-      if (rc) return rc;
-    }
+    // asks for the last force: enqueued and checked by the next settle,
+    // which redoes it on the radix path after a device-side failure
+    if (int rc = warm_interval(s, h, nleap)) return rc;
+  } else {
+    if (int rc = cold_interval(s, h, nleap, acc_last, cum_last)) return rc;
   }
   WDY_HIP(hipEventRecord(s->ev1, s->stream));
   return WDY_OK;
@@ -338,6 +437,12 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
 static int check_sim(wdy_sim *s, const char *fn) {
   if (!s) return fail(WDY_EINVAL, "%s: sim is NULL", fn);
   return WDY_OK;
+}
+
+// check_sim, then settle the warm interval in flight
+static int check_settled(wdy_sim *s, const char *fn) {
+  if (int rc = check_sim(s, fn)) return rc;
+  return settle(s);
 }
 
 }  // namespace wdy
@@ -373,7 +478,7 @@ int wdy_advance(wdy_sim *sim, double h, int nleap) {
 }
 
 int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out) {
-  if (int rc = check_sim(sim, "wdy_read")) return rc;
+  if (int rc = check_settled(sim, "wdy_read")) return rc;
   WDY_HIP(hipSetDevice(sim->device));
   const size_t N = (size_t)sim->n;
   if (x_out || v_out) {
@@ -398,7 +503,7 @@ int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out) {
 
 int wdy_read_and_advance(wdy_sim *sim, double *x_out, double *v_out, double h,
                          int nleap, double *seconds_out) {
-  if (int rc = check_sim(sim, "wdy_read_and_advance")) return rc;
+  if (int rc = check_settled(sim, "wdy_read_and_advance")) return rc;
   if (nleap < 1) return fail(WDY_EINVAL, "wdy_read_and_advance: nleap=%d < 1", nleap);
   if (!std::isfinite(h)) return fail(WDY_EINVAL, "wdy_read_and_advance: h not finite");
   if (sim->small) {  // velocities live in the state buffer: read, then advance
@@ -445,7 +550,7 @@ int wdy_host_free(void *p) {
 }
 
 int wdy_energy(wdy_sim *sim, double twopiG, double omega2, double *e_out) {
-  if (int rc = check_sim(sim, "wdy_energy")) return rc;
+  if (int rc = check_settled(sim, "wdy_energy")) return rc;
   if (!e_out) return fail(WDY_EINVAL, "wdy_energy: e_out is NULL");
   if (!(twopiG != 0.0) || !std::isfinite(twopiG))
     return fail(WDY_EINVAL, "wdy_energy: twopiG must be finite and nonzero");
@@ -462,7 +567,7 @@ int wdy_destroy(wdy_sim *sim) {
 }
 
 int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {
-  if (int rc = check_sim(sim, "wdy_tune")) return rc;
+  if (int rc = check_settled(sim, "wdy_tune")) return rc;
   if (sim->small || !sim->warm)
     return fail(WDY_EUNSUPPORTED, "wdy_tune: the sim has no warm-start path");
   if (shape < -1 || shape >= WARM_NSHAPES)
@@ -480,7 +585,7 @@ int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {
 }
 
 int wdy_set_ext_force(wdy_sim *sim, int kind, const double *params, double t0) {
-  if (int rc = check_sim(sim, "wdy_set_ext_force")) return rc;
+  if (int rc = check_settled(sim, "wdy_set_ext_force")) return rc;
   if (kind < WDY_EXT_NONE || kind > WDY_EXT_PULSE)
     return fail(WDY_EINVAL, "wdy_set_ext_force: unknown kind %d", kind);
   if (kind != WDY_EXT_NONE && !params)
@@ -502,12 +607,31 @@ int wdy_set_ext_force(wdy_sim *sim, int kind, const double *params, double t0) {
   WDY_HIP(hipSetDevice(sim->device));
   WDY_HIP(hipStreamSynchronize(sim->stream));
   sim->ext = f;
+  sim->host_force = nullptr;
+  sim->t0 = t0;
+  return WDY_OK;
+}
+
+int wdy_set_ext_force_host(wdy_sim *sim,
+                           double (*force)(int, double *, double, double *),
+                           double t0) {
+  if (int rc = check_settled(sim, "wdy_set_ext_force_host")) return rc;
+  if (!std::isfinite(t0))
+    return fail(WDY_EINVAL, "wdy_set_ext_force_host: t0 not finite");
+  if (force && sim->small)
+    return fail(WDY_EUNSUPPORTED,
+                "wdy_set_ext_force_host: the simulation runs on the small path; "
+                "create it with WDY_FORCE_LARGE");
+  WDY_HIP(hipSetDevice(sim->device));
+  WDY_HIP(hipStreamSynchronize(sim->stream));
+  sim->ext = ExtForce();
+  sim->host_force = force;
   sim->t0 = t0;
   return WDY_OK;
 }
 
 int wdy_time(wdy_sim *sim, double *t_out) {
-  if (int rc = check_sim(sim, "wdy_time")) return rc;
+  if (int rc = check_settled(sim, "wdy_time")) return rc;
   if (!t_out) return fail(WDY_EINVAL, "wdy_time: t_out is NULL");
   *t_out = sim->t0;
   return WDY_OK;
@@ -515,7 +639,7 @@ int wdy_time(wdy_sim *sim, double *t_out) {
 
 int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
                    int64_t *redos) {
-  if (int rc = check_sim(sim, "wdy_path_stats")) return rc;
+  if (int rc = check_settled(sim, "wdy_path_stats")) return rc;
   if (warm_steps) *warm_steps = sim->warm_steps;
   if (cold_steps) *cold_steps = sim->cold_steps;
   if (redos) *redos = sim->redos;
@@ -524,7 +648,7 @@ int wdy_path_stats(wdy_sim *sim, int64_t *warm_steps, int64_t *cold_steps,
 
 int wdy_warm_failures(wdy_sim *sim, int64_t *capacity, int64_t *overflow,
                       int64_t *stall) {
-  if (int rc = check_sim(sim, "wdy_warm_failures")) return rc;
+  if (int rc = check_settled(sim, "wdy_warm_failures")) return rc;
   if (capacity) *capacity = sim->fail_capacity;
   if (overflow) *overflow = sim->fail_overflow;
   if (stall) *stall = sim->fail_stall;
@@ -532,7 +656,7 @@ int wdy_warm_failures(wdy_sim *sim, int64_t *capacity, int64_t *overflow,
 }
 
 int wdy_warm_config(wdy_sim *sim, int *shape, int *bucket_target, int *window) {
-  if (int rc = check_sim(sim, "wdy_warm_config")) return rc;
+  if (int rc = check_settled(sim, "wdy_warm_config")) return rc;
   int sh = -1;
   u32 T = 0, W = 0;
   if (!warm_config(sim, &sh, &T, &W)) sh = -1;
@@ -621,19 +745,30 @@ void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
                                                      double *),
                                  int sort_type, int *err,
                                  double *time_elapsed, double *cumulmass) {
-  (void)t0;
+  // The reference's signature has no error path on this call (err is never
+  // written there): a failure here sets *err, leaves a message for
+  // wdy_last_error() and is also reported on stderr, so that a caller that
+  // ignores err still sees it.
+  // The positions and velocities handed back after a failure are NaN, so
+  // that a run that goes on regardless does not continue from a state that
+  // silently stopped advancing.
   auto set_err = [&](int code) {
     if (err) *err = code;
+    if (!code) return;
+    fprintf(stderr, "wendy_amd: _wendy_nbody_approx_onestep failed (%d): %s\n",
+            code, wdy_last_error());
+    const double nan = std::nan("");
+    for (int i = 0; i < N; ++i) {
+      if (x) x[i] = nan;
+      if (v) v[i] = nan;
+      if (xi) xi[i].val = nan;
+    }
   };
-  if (ext_force) {
-    fail(WDY_EUNSUPPORTED, "ext_force is not supported on the GPU path");
-    return set_err(WDY_EUNSUPPORTED);
-  }
   if (sort_type < 0 || sort_type > 4) {
     fail(WDY_EINVAL, "unknown sort_type %d", sort_type);
     return set_err(WDY_EINVAL);
   }
-  if (N < 1 || !xi || !x || !v || !m || nleap < 1) {
+  if (N < 1 || !xi || !x || !v || !m || nleap < 1 || (ext_force && !t0)) {
     fail(WDY_EINVAL, "invalid arguments");
     return set_err(WDY_EINVAL);
   }
@@ -654,10 +789,18 @@ void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
     eq = eq && (m[i] == m[0]);
   }
   wdy_sim *s = nullptr;
-  // the reference hands xi back in sorted order, which the radix path keeps
+  // the reference hands xi back in sorted order, which the radix path keeps;
+  // an external force is the caller's host function, called before every
+  // step as the reference calls it (radix path)
   int rc = create_impl(N, xs.data(), ids.data(), v, m, totmass, omega2,
-                       (eq ? WDY_EQUAL_MASS : 0) | WDY_NO_WARM, &s);
+                       (eq ? WDY_EQUAL_MASS : 0) | WDY_NO_WARM |
+                           (ext_force ? WDY_FORCE_LARGE : 0),
+                       &s);
   if (rc) return set_err(rc);
+  if (ext_force) {
+    s->host_force = ext_force;
+    s->t0 = *t0;
+  }
   double *acc = nullptr, *acc_o = nullptr, *cum_o = nullptr;
   rc = [&]() -> int {
     WDY_HIP(hipMalloc(&acc, n * 8));
@@ -691,6 +834,7 @@ void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
     float ms_ = 0.f;
     if (hipEventElapsedTime(&ms_, s->ev0, s->ev1) != hipSuccess) ms_ = 0.f;
     if (time_elapsed) *time_elapsed = ms_ * 1e-3;
+    if (ext_force) *t0 = s->t0;
     return WDY_OK;
   }();
   if (acc) (void)hipFree(acc);

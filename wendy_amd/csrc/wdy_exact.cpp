@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <ctime>
 #include <cstdint>
 #include <new>
 #include <numeric>
@@ -165,12 +166,16 @@ struct wdy_exact {
   }
 
   // One output interval of length dt; returns the number of crossings.
-  int64_t step(double dt, int64_t maxcoll, bool *hit_max) {
+  // first >= 0 names the pair that crosses first (a caller that keeps its
+  // own record of the earliest crossing, as the reference's interface does);
+  // later crossings come from the heap.
+  int64_t step(double dt, int64_t maxcoll, bool *hit_max, int first = -1) {
     std::fill(t.begin(), t.end(), 0.0);
     int64_t cnt = 0;
-    while (!heap.empty() && tc[heap.top()] < dt && cnt < maxcoll) {
+    while (!heap.empty() && cnt < maxcoll) {
+      const int k = (cnt == 0 && first >= 0) ? first : heap.top();
+      if (!(tc[k] < dt)) break;
       ++cnt;
-      const int k = heap.top();
       const double tn = tc[k];
       const int i = ord[k], j = ord[k + 1];  // i below j until now
       fly(i, tn);
@@ -279,6 +284,96 @@ int wdy_exact_read(wdy_exact *e, double *x_out, double *v_out) {
 int wdy_exact_destroy(wdy_exact *e) {
   delete e;
   return WDY_OK;
+}
+
+// ---- reference-signature entry points --------------------------------------
+// The caller owns the whole solver state between calls, as in the reference:
+// x, v, a per sheet, sindx (sheet at each position rank), tcoll (crossing
+// time of ranks k, k+1, relative to the start of the call), cindx and
+// next_tcoll (the earliest crossing).  One call runs one output interval dt
+// and hands the state back shifted to the interval's end.
+
+double _solve_coll_quad(double c0, double c1, double c2) {
+  return cross_quad(c0, c1, c2);
+}
+
+double _solve_coll_harm(double c0, double c1, double c2, double omega) {
+  return cross_harm(c0, c1, c2, omega);
+}
+
+}  // extern "C"
+
+namespace {
+
+void exact_compat_step(int N, double *x, double *v, double *a, double *m,
+                       int *sindx, int *cindx, double *next_tcoll,
+                       double *tcoll, double dt, int maxcoll, int *err,
+                       int *ncoll, double *time_elapsed, double omega) {
+  if (N < 1 || !x || !v || !a || !m || !sindx || !cindx || !next_tcoll ||
+      (N > 1 && !tcoll)) {
+    wdy::fail(WDY_EINVAL, "exact step: invalid arguments");
+    if (err) *err = WDY_EINVAL;
+    return;
+  }
+  wdy_exact e;
+  e.n = N;
+  e.omega = omega;
+  e.x.assign(x, x + N);
+  e.v.assign(v, v + N);
+  e.a.assign(a, a + N);
+  e.m.assign(m, m + N);
+  e.t.assign(N, 0.0);
+  e.ord.assign(sindx, sindx + N);
+  e.tc.assign(tcoll, tcoll + (N - 1));
+  e.heap.build(&e.tc);
+  const int first = (N > 1 && *cindx >= 0 && *cindx < N - 1) ? *cindx : -1;
+  bool hit = false;
+  // the reference times the crossing loop with the process CPU clock
+  const std::clock_t c0 = std::clock();
+  const int64_t cnt = e.step(dt, maxcoll, &hit, first);
+  const std::clock_t c1 = std::clock();
+  std::copy(e.x.begin(), e.x.end(), x);
+  std::copy(e.v.begin(), e.v.end(), v);
+  std::copy(e.a.begin(), e.a.end(), a);
+  std::copy(e.ord.begin(), e.ord.end(), sindx);
+  std::copy(e.tc.begin(), e.tc.end(), tcoll);
+  // the earliest pending crossing, shifted like tcoll; unchanged (but
+  // shifted) when nothing crossed
+  if (cnt > 0 && !e.heap.empty()) {
+    *cindx = e.heap.top();
+    *next_tcoll = e.tc[*cindx];
+  } else {
+    *next_tcoll -= dt;
+  }
+  if (ncoll) *ncoll = (int)cnt;
+  if (time_elapsed) *time_elapsed = (double)(c1 - c0) / CLOCKS_PER_SEC;
+  if (hit && err) *err = -2;
+}
+
+}  // namespace
+
+extern "C" {
+
+void _wendy_nbody_onestep(int N, double *x, double *v, double *a, double *m,
+                          int *sindx, int *cindx, double *next_tcoll,
+                          double *tcoll, double dt, int maxcoll, int *err,
+                          int *ncoll, double *time_elapsed) {
+  exact_compat_step(N, x, v, a, m, sindx, cindx, next_tcoll, tcoll, dt,
+                    maxcoll, err, ncoll, time_elapsed, 0.0);
+}
+
+void _wendy_nbody_harm_onestep(int N, double *x, double *v, double *a,
+                               double *m, int *sindx, int *cindx,
+                               double *next_tcoll, double *tcoll, double dt,
+                               int maxcoll, int *err, int *ncoll,
+                               double *time_elapsed, double omega) {
+  if (!(omega > 0.0)) {
+    wdy::fail(WDY_EINVAL, "_wendy_nbody_harm_onestep: omega must be > 0");
+    if (err) *err = WDY_EINVAL;
+    return;
+  }
+  exact_compat_step(N, x, v, a, m, sindx, cindx, next_tcoll, tcoll, dt,
+                    maxcoll, err, ncoll, time_elapsed, omega);
 }
 
 }  // extern "C"

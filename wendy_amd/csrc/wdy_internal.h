@@ -56,7 +56,11 @@ __host__ __device__ inline double decode_pos(u64 k) {
 struct ExtForce {
   int kind = 0;  // WDY_EXT_* of include/wendy_amd.h; 0: none
   double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  // EXT_TABLE: acceleration of sheet j in tab[j], filled by the host from a
+  // caller's function before each step (radix-path steps only)
+  const double *tab = nullptr;
 };
+constexpr int EXT_TABLE = 4;
 
 __device__ inline double ext_accel(const ExtForce &f, double x, double t) {
   switch (f.kind) {
@@ -71,6 +75,12 @@ __device__ inline double ext_accel(const ExtForce &f, double x, double t) {
     default:
       return 0.0;
   }
+}
+
+// The same for a kernel that knows the sheet index (the per-sheet table).
+__device__ inline double ext_accel_id(const ExtForce &f, double x, double t,
+                                      i32 id) {
+  return f.kind == EXT_TABLE ? f.tab[id] : ext_accel(f, x, t);
 }
 
 // Error reporting: every C-ABI entry point returns one of the WDY_E* codes and
@@ -182,6 +192,12 @@ struct wdy_sim {
   // evaluation while an external force is set (wendy/wendy.c:401-402,407-408)
   wdy::ExtForce ext;
   double t0 = 0.0;
+  // external force given as a host function with the reference's callback
+  // signature (wendy/wendy.c:362-370), evaluated before every step on the
+  // positions copied out; ext_tab holds its values on the device (and the
+  // positions it is called on)
+  double (*host_force)(int, double *, double, double *) = nullptr;
+  double *ext_tab = nullptr;
   // ranks the fastest sheets cross per unit time, estimated from a sample
   // at creation (sizes the warm-start window once h is known)
   double disp_rate = 0.0;
@@ -233,7 +249,13 @@ int warm_partition_after_cold(wdy_sim *s, double hp);
 // nothing after wdy_tune or once done.
 int warm_fit(wdy_sim *s, double h);
 void warm_mark_tuned(wdy_sim *s);
-int warm_interval(wdy_sim *s, double h, int nleap, bool *failed);
+// the state was advanced outside the warm path: its layout must be rebuilt
+void warm_invalidate(wdy_sim *s);
+// enqueue one warm output interval (checked later by warm_settle)
+int warm_interval(wdy_sim *s, double h, int nleap);
+// wait for and check the warm interval in flight; *failed: it was rewound
+// and (h, nleap) must be redone on the radix path
+int warm_settle(wdy_sim *s, bool *failed, double *h, int *nleap);
 int warm_current_shape(wdy_sim *s);
 // the warm path's tile shape, bucket target and window; false without one
 bool warm_config(wdy_sim *s, int *shape, u32 *T, u32 *W);

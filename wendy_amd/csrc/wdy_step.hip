@@ -113,7 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_leap(
     C = tied_prefix<EQ>(ks, perm, idx_old, mass_old, cum, n, i, k, id, mq);
   double F = (totmass - 2.0 * C) - m;
   if (HARM) F = F - omega2 * x;
-  const double a = (ext.kind ? ext_accel(ext, x, t) : 0.0) + F;
+  const double a = (ext.kind ? ext_accel_id(ext, x, t, id) : 0.0) + F;
   v = v + hk * a;
   x = x + hd * v;
   knew[i] = encode_pos(x);

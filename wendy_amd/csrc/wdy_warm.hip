@@ -1056,6 +1056,13 @@ struct WarmCtx {
   u64 *status = nullptr;
   u32 *err = nullptr;
   u32 *h_err = nullptr;  // pinned copy of err
+  // the last warm interval, enqueued but not yet checked: its error word is
+  // copied to h_err and ev_err marks the copy's completion; until checked,
+  // the interval's start set and metadata (p_start, p_mstart) stay intact
+  hipEvent_t ev_err = nullptr;
+  bool pending = false;
+  int p_start = 0, p_mstart = 0, p_nleap = 0;
+  double p_h = 0.0, p_t0 = 0.0;
   u32 epoch = 0;
   bool ready = false;    // meta[rd] partitions the current state set
   double hp = 0.0;       // half drift the partition anticipates
@@ -1116,6 +1123,7 @@ void warm_free(wdy_sim *s) {
   if (w->err) (void)hipFree(w->err);
   if (w->ring) (void)hipFree(w->ring);
   if (w->h_err) (void)hipHostFree(w->h_err);
+  if (w->ev_err) (void)hipEventDestroy(w->ev_err);
   delete w;
   s->warm = nullptr;
 }
@@ -1131,6 +1139,7 @@ int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
     WDY_HIP(hipMalloc(&s->warm->ring, 32));
     WDY_HIP(hipMemset(s->warm->ring, 0, 32));
     WDY_HIP(hipHostMalloc((void **)&s->warm->h_err, 8, hipHostMallocDefault));
+    WDY_HIP(hipEventCreateWithFlags(&s->warm->ev_err, hipEventDisableTiming));
     // fixed-point scale: 2^k * totmass <= 2^62
     int e = 0;
     (void)frexp(s->totmass > 0.0 ? s->totmass : 1.0, &e);
@@ -1281,6 +1290,10 @@ void warm_mark_tuned(wdy_sim *s) {
   if (s->warm) s->warm->tuned = true;
 }
 
+void warm_invalidate(wdy_sim *s) {
+  if (s->warm) s->warm->ready = false;
+}
+
 int warm_fit(wdy_sim *s, double h) {
   WarmCtx &w = *s->warm;
   if (w.fitted || w.tuned) return WDY_OK;
@@ -1327,12 +1340,15 @@ bool warm_ready_for(wdy_sim *s, double h) {
   return s->warm && s->warm->ready && s->warm->hp == h / 2.;
 }
 
-// One output interval of nleap warm steps.  On a device-side failure the
-// state is rewound to the interval's start (never written by the interval),
-// the bucket target or window is adapted, and *failed is set.
-int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
+// Enqueues one output interval of nleap warm steps and returns without
+// waiting for it: the device's error word is copied to the host behind the
+// interval and checked by warm_settle.  The state and metadata sets the
+// interval starts from are not written by it (three sets rotate), so a
+// failure found later can still rewind to them.
+int warm_interval(wdy_sim *s, double h, int nleap) {
   WarmCtx &w = *s->warm;
-  *failed = false;
+  if (w.pending) return fail(WDY_ERUNTIME, "warm interval enqueued before "
+                             "the previous one was checked");
   const int start = s->cur, mstart = w.rd;
   int so[2], mo[2], ns = 0, nm = 0;
   for (int i = 0; i < 3; ++i) {
@@ -1368,56 +1384,81 @@ int warm_interval(wdy_sim *s, double h, int nleap, bool *failed) {
     rdm = wrm;
   }
   WDY_HIP(hipMemcpyAsync(w.h_err, w.err, 8, hipMemcpyDeviceToHost, s->stream));
-  WDY_HIP(hipStreamSynchronize(s->stream));
-  if (w.prof && w.h_err[0] == 0) prof_fold(s);
-  if (w.h_err[0] != 0) {
-    const u32 kind = w.h_err[1];
-    *failed = true;
-    if (kind & WARM_ERR_CAPACITY) s->fail_capacity += 1;
-    if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) s->fail_overflow += 1;
-    if (kind & WARM_ERR_STALL) s->fail_stall += 1;
-    s->cur = start;
-    w.rd = mstart;
-    w.ready = false;
-    if (kind & WARM_ERR_STALL) {  // not a sizing problem: stop using it
-      warm_free(s);
-      return WDY_OK;
-    }
-    u32 T = w.T, W = w.W;
-    int shape = w.shape;
-    if (kind & WARM_ERR_CAPACITY) T = T * 15 / 16 < WARM_T_MIN ? WARM_T_MIN : T * 15 / 16;
-    if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) {
-      // sheets cross more buckets per step than the window holds: widen it,
-      // and once it is at its limit move to larger buckets (2048 sheets,
-      // then 4096), as the window fitting does
-      const u32 wmax = (u32)warm_shape(shape).wmax();
-      if (W * 2 + 1 <= wmax) {
-        W = W * 2 + 1;
-      } else if (warm_shape(shape).cap < warm_shape(WARM_SHAPE_WIDE).cap) {
-        shape = WARM_SHAPE_WIDE;
-        T = 0;
-        const u32 wm = (u32)warm_shape(shape).wmax();
-        W = W < wm ? W : wm;
-      } else if (warm_shape(shape).cap < warm_shape(0).cap) {
-        shape = 0;
-        T = 0;
-        const u32 wm = (u32)warm_shape(shape).wmax();
-        W = W < wm ? W : wm;
-      } else {
-        W = wmax;
-      }
-    }
-    if (shape == w.shape && T == w.T && W == w.W) {  // limits reached: radix
-      warm_free(s);                                  // path from now on
-      return WDY_OK;
-    }
-    if (int rc = warm_configure(s, shape, T, W)) return rc;
-    return WDY_OK;
-  }
+  WDY_HIP(hipEventRecord(w.ev_err, s->stream));
+  w.pending = true;
+  w.p_start = start;
+  w.p_mstart = mstart;
+  w.p_h = h;
+  w.p_nleap = nleap;
+  w.p_t0 = s->t0;
+  // the state as it will be when the interval succeeds
   s->cur = in;
   w.rd = rdm;
   if (s->ext.kind) s->t0 = t;
   return WDY_OK;
+}
+
+// Waits for the warm interval in flight, if any, and checks its error word.
+// On a device-side failure the state is rewound to the interval's start, the
+// bucket target or window is adapted (or the warm path dropped), and
+// *failed is set with the interval's h and nleap, which the caller must redo
+// on the radix path.
+int warm_settle(wdy_sim *s, bool *failed, double *h, int *nleap) {
+  *failed = false;
+  WarmCtx *wp = s->warm;
+  if (!wp || !wp->pending) return WDY_OK;
+  WarmCtx &w = *wp;
+  WDY_HIP(hipEventSynchronize(w.ev_err));
+  w.pending = false;
+  if (w.h_err[0] == 0) {
+    if (w.prof) prof_fold(s);
+    s->warm_steps += w.p_nleap;
+    return WDY_OK;
+  }
+  const u32 kind = w.h_err[1];
+  *failed = true;
+  *h = w.p_h;
+  *nleap = w.p_nleap;
+  if (kind & WARM_ERR_CAPACITY) s->fail_capacity += 1;
+  if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) s->fail_overflow += 1;
+  if (kind & WARM_ERR_STALL) s->fail_stall += 1;
+  s->cur = w.p_start;
+  s->t0 = w.p_t0;
+  w.rd = w.p_mstart;
+  w.ready = false;
+  if (kind & WARM_ERR_STALL) {  // not a sizing problem: stop using it
+    warm_free(s);
+    return WDY_OK;
+  }
+  u32 T = w.T, W = w.W;
+  int shape = w.shape;
+  if (kind & WARM_ERR_CAPACITY) T = T * 15 / 16 < WARM_T_MIN ? WARM_T_MIN : T * 15 / 16;
+  if (kind & (WARM_ERR_OVF | WARM_ERR_LOVF)) {
+    // sheets cross more buckets per step than the window holds: widen it,
+    // and once it is at its limit move to larger buckets (2048 sheets,
+    // then 4096), as the window fitting does
+    const u32 wmax = (u32)warm_shape(shape).wmax();
+    if (W * 2 + 1 <= wmax) {
+      W = W * 2 + 1;
+    } else if (warm_shape(shape).cap < warm_shape(WARM_SHAPE_WIDE).cap) {
+      shape = WARM_SHAPE_WIDE;
+      T = 0;
+      const u32 wm = (u32)warm_shape(shape).wmax();
+      W = W < wm ? W : wm;
+    } else if (warm_shape(shape).cap < warm_shape(0).cap) {
+      shape = 0;
+      T = 0;
+      const u32 wm = (u32)warm_shape(shape).wmax();
+      W = W < wm ? W : wm;
+    } else {
+      W = wmax;
+    }
+  }
+  if (shape == w.shape && T == w.T && W == w.W) {  // limits reached: radix
+    warm_free(s);                                  // path from now on
+    return WDY_OK;
+  }
+  return warm_configure(s, shape, T, W);
 }
 
 }  // namespace wdy

@@ -177,12 +177,23 @@ class Sim:
         return ms.value, cnt.value
 
     def set_ext_force(self, force, t0=0.):
-        """Evaluate ``force`` (a wendy_amd.forces.ExtForce, or None) at every
-        step from the next ``advance``, with the clock set to ``t0``."""
+        """Evaluate ``force`` at every step from the next ``advance``, with
+        the clock set to ``t0``: a wendy_amd.forces.ExtForce is evaluated on
+        the device inside the step kernels; any other callable ``f(x, t)`` is
+        called on the host before every step, as the reference calls its
+        ``ext_force`` (the simulation must be on the large path); None
+        removes the force."""
+        if force is not None and not isinstance(force, ExtForce):
+            cb = _lib.ext_force_callback(force)
+            _lib.check(_lib.lib().wdy_set_ext_force_host(
+                self._h, ctypes.cast(cb, ctypes.c_void_p), float(t0)))
+            self._ext_cb = cb  # the C side holds its address
+            return
         kind = 0 if force is None else force.kind
         p = numpy.array(force.params() if force is not None else (0., 0., 0.),
                         dtype=numpy.float64)
         _lib.check(_lib.lib().wdy_set_ext_force(self._h, kind, p, float(t0)))
+        self._ext_cb = None
 
     def time(self):
         """The clock t0 (advances only while an external force is set)."""
@@ -223,11 +234,13 @@ def nbody(x, v, m, dt, t0=0., twopiG=1., omega=None, ext_force=None,
     and the device computes the next interval while the caller works on the
     current one.
 
-    ``ext_force`` must be one of the device-evaluated forms of
-    ``wendy_amd.forces`` (Harmonic, Isothermal, Pulse), which are also plain
-    callables ``f(x, t)``; the force is evaluated at the clock ``t0``, which
-    advances by dt/nleap after every force evaluation, as in the reference.
-    Other callables raise NotImplementedError at the first ``next()``.
+    ``ext_force`` is any callable ``f(x, t)``, evaluated at the clock
+    ``t0``, which advances by dt/nleap after every force evaluation, as in
+    the reference.  The forms of ``wendy_amd.forces`` (Harmonic, Isothermal,
+    Pulse) are evaluated on the device inside the step kernels; any other
+    callable is called on the host before every leapfrog step with the
+    positions of that step, as the reference calls it (one device-host round
+    trip per step; the state stays on the device).
     ``approx=False`` runs the exact event-driven solver on the CPU
     (libwendy_amd's wdy_exact_*), the reference's own machine-precision
     method, with its checks and its ``maxcoll``/``warn_maxcoll`` behaviour;
@@ -301,11 +314,7 @@ def _nbody_exact(x, v, m, dt, twopiG=1., omega=None, ext_force=None,
 def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
                   sort='quick', twopiG=1., full_output=False):
     """Approx path (wendy/wendy.py:336-437) with device-resident state."""
-    if ext_force is not None and not isinstance(ext_force, ExtForce):
-        raise NotImplementedError(
-            'ext_force must be one of wendy_amd.forces.Harmonic, Isothermal, '
-            'Pulse (evaluated on the GPU); arbitrary Python callables are not '
-            'supported')
+    host_force = ext_force is not None and not isinstance(ext_force, ExtForce)
     _sort_type_dict[sort]  # unknown names raise KeyError, as the reference
     if omega is None:
         omega2 = -1.
@@ -318,7 +327,9 @@ def _nbody_approx(x, v, m, dt, nleap, t0=0., omega=None, ext_force=None,
     v = numpy.require(v, dtype=numpy.float64, requirements=['C', 'W'])
     m = numpy.require(m, dtype=numpy.float64, requirements=['C', 'W'])
     totmass = numpy.sum(m)
-    sim = Sim(x, v, m, totmass, omega2)
+    # a host-evaluated force runs on the large path's per-step launches
+    sim = Sim(x, v, m, totmass, omega2,
+              path='large' if host_force else None, warm=not host_force)
     if ext_force is not None:
         sim.set_ext_force(ext_force, t0)
     dt_leap = dt / nleap
