@@ -253,39 +253,79 @@ def bench_serve(args, log):
             'wall_s': s.stats['wall_s']}
 
 
-def cpu_baseline(x, v, m, dt, nleap, omega, threads, budget_s=25.):
+def host_cpus():
+    """(CPU model, logical CPUs of the machine, CPUs this process may run on,
+    threads to use: the latter capped by OMP_NUM_THREADS when it is set)."""
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    total = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = total
+    use = avail
+    env = os.environ.get('OMP_NUM_THREADS', '')
+    if env.isdigit() and int(env) > 0:
+        use = min(use, int(env))
+    return model, total, avail, use
+
+
+REF_TEAMS = (1, 2, 4, 8, 16, 32, 64, 128)
+
+
+def cpu_baseline(x, v, m, dt, nleap, omega, threads=None, budget_s=25.):
     """Time the reference C (sort='parallel') on a bounded sample.
 
-    One warm-up output (it contains the cold sort of the random initial
-    order), then outputs of ``k`` leapfrog steps until ``budget_s`` of CPU
-    work; rate = N * steps / time_elapsed, using the reference's own in-C
-    wall clock (wendy/wendy.c:396,416-417)."""
+    The reference fixes its parallel-sort team size at compile time
+    (wendy/parallel_sort.h:13-15); oracle/Makefile builds one library per
+    team size, and the one for the threads used here is loaded, with
+    OMP_NUM_THREADS set to the same count.  One warm-up output (it contains
+    the cold sort of the random initial order), then outputs of ``nleap``
+    leapfrog steps (the workload's own) until ``budget_s`` of CPU work;
+    rate = N * steps / time_elapsed, using the reference's own in-C wall
+    clock (wendy/wendy.c:396,416-417)."""
     from oracle import ref_harness as R
-    if not R.available():
+    model, total, avail, use = host_cpus()
+    if threads is not None:
+        use = threads
+    team = max(k for k in REF_TEAMS if k <= max(1, use))
+    variant = '_t%d' % team
+    if not R.available(variant):
         return None
-    os.environ['OMP_NUM_THREADS'] = str(threads)
+    os.environ['OMP_NUM_THREADS'] = str(team)
     n = len(x)
     h = dt / nleap
-    k = 2
-    ref = R.RefApprox(x, v, m, h * k, k, omega=omega, sort='parallel')
+    ref = R.RefApprox(x, v, m, h * nleap, nleap, omega=omega, sort='parallel',
+                      variant=variant)
     t0 = time.time()
     ref.onestep()
     warm = time.time() - t0
     steps, secs = 0, 0.
-    while secs < budget_s and (time.time() - t0) < budget_s + warm + 5:
+    while secs < budget_s and (time.time() - t0) < budget_s + warm + 30:
         _, _, te = ref.onestep()
-        steps += k
+        steps += nleap
         secs += te
     if steps == 0:
         return None
     return {'value': n * steps / secs, 'unit': 'sheet-steps/s',
-            'cores': threads, 'kind': 'reference',
+            'cores': team, 'kind': 'reference',
+            'cpu_model': model, 'nproc': total, 'cpus_available': avail,
             'sample': ('N=%d sheets of this workload, reference C '
                        "_wendy_nbody_approx_onestep sort='parallel' "
-                       '(OpenMP, PARALLEL_SORT_NUM_THREADS=16, '
-                       'OMP_NUM_THREADS=%d), %d leapfrog steps after one '
-                       'discarded cold-sort output, %.1f s timed by the '
-                       "reference's own wall clock" % (n, threads, steps, secs))}
+                       '(OpenMP, PARALLEL_SORT_NUM_THREADS=%d, '
+                       'OMP_NUM_THREADS=%d) on %d of the %d CPUs this process '
+                       'may use (%s, %d logical CPUs in the machine): %d '
+                       'outputs of nleap=%d leapfrog steps after one discarded '
+                       "cold-sort output, %.1f s timed by the reference's own "
+                       'wall clock' % (n, team, team, team, avail, model, total,
+                                       steps // nleap, nleap, secs))}
 
 
 def measured_traffic(name):
@@ -304,18 +344,125 @@ def measured_traffic(name):
 
 
 def copy_probe(mib=1024, launches=5):
-    """A plain device copy of known size (torch elementwise kernel: read
+    """A plain device copy of known size (libwendy_amd's wdy_copy_probe: read
     mib MiB, write mib MiB per launch) whose HBM counters calibrate the
     FETCH_SIZE / WRITE_SIZE readings of the kernels above."""
-    import torch
-    n = mib * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device='cuda')
-    b = torch.empty_like(a)
-    for _ in range(launches):
-        torch.add(a, 1., out=b)
-    torch.cuda.synchronize()
-    print(json.dumps({'copy_probe_read_bytes': n * 8, 'copy_probe_write_bytes': n * 8,
-                      'launches': launches}))
+    import ctypes
+    from wendy_amd import _lib
+    ms = ctypes.c_double(0.)
+    nbytes = mib * (1 << 20)
+    _lib.check(_lib.lib().wdy_copy_probe(nbytes, launches, ctypes.byref(ms)))
+    print(json.dumps({'copy_probe_read_bytes': nbytes,
+                      'copy_probe_write_bytes': nbytes, 'launches': launches,
+                      'ms_per_launch': ms.value,
+                      'GB_per_s': 2 * nbytes / (ms.value * 1e-3) / 1e9}))
+
+
+def bench_plain(name, steps, warmup, log, e2e=True):
+    """One simulation of workload ``name`` on the GPU: ``warmup`` output
+    intervals, then ``steps`` timed intervals of device work bracketed by
+    synchronisations (the rate), a second timed region with an event pair
+    around every launch (the dominant kernel's time per launch), and with
+    ``e2e`` the generator's loop in steady state."""
+    import wendy_amd
+    from wendy_amd import _lib as wlib
+    wl = WORKLOADS[name]
+    n, nleap, dt, omega = wl['n'], wl['nleap'], wl['dt'], wl['omega']
+    x, v, m = workload(n, lognormal=wl['lognormal'], vshift=wl['vshift'])
+    equal = wl['lognormal'] is None
+    h = dt / nleap
+    log('bench %s: n=%d nleap=%d steps=%d warmup=%d' % (name, n, nleap, steps,
+                                                        warmup))
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.)
+    del x, v, m
+    for _ in range(warmup):
+        sim.advance(h, nleap)
+    sim.read()
+    log('bench %s: warmup done' % name)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.advance(h, nleap)
+    sim.read()  # synchronise
+    elapsed = time.perf_counter() - t0
+    rate = n * nleap * steps / elapsed
+    log('bench %s: device-only %.3e sheet-steps/s (%.2f ms/output)'
+        % (name, rate, 1e3 * elapsed / steps))
+    # per-kernel device time: a second timed region of the same work with a
+    # HIP event pair around every launch on the simulation's stream (the
+    # events themselves add a little time between launches, so the rate
+    # above is taken without them)
+    kb = max(1, min(steps, 20))
+    sim.set_timing(True)
+    for _ in range(kb):
+        sim.advance(h, nleap)
+    sim.read()
+    sim.set_timing(False)
+    kt = {c: sim.kernel_time(c) for c in range(6)}
+    ps = sim.path_stats()
+    paths = {'warm_steps': ps[0], 'cold_steps': ps[1], 'redos': ps[2],
+             'failures_capacity_overflow_stall': list(sim.warm_failures())}
+    shape, target, window = sim.warm_config()
+    log('bench %s: paths %s, warm shape %d target %d window %d'
+        % (name, paths, shape, target, window))
+    e2e_rate = None
+    if e2e:
+        # end-to-end as the nbody generator runs it: each output is de-sorted
+        # and copied to page-locked host arrays while the next interval runs
+        xo = wlib.pinned_empty(n)
+        vo = wlib.pinned_empty(n)
+        ne = max(1, min(10, steps))
+        sim.read()
+        sim.advance(h, nleap)
+        sim.read_and_advance(xo, vo, h, nleap)  # the pipeline is full from here
+        t0 = time.perf_counter()
+        for _ in range(ne):
+            sim.read_and_advance(xo, vo, h, nleap)
+        e2e_rate = n * nleap * ne / (time.perf_counter() - t0)
+        log('bench %s: end-to-end %.3e sheet-steps/s' % (name, e2e_rate))
+    sim.close()
+
+    # dominant kernel and its algorithmic bytes per launch
+    busy = {c: kt[c][0] for c in kt}
+    dom = max(busy, key=busy.get)
+    ms, cnt = kt[dom]
+    if dom == KC_SORT_PASS:
+        kname, bpl = 'radix scatter pass', 24 * n
+        note = 'read key 8 B + payload 4 B, write key 8 B + payload 4 B per sheet'
+    elif dom == KC_WARM:
+        per = 40 if equal else 56
+        kname, bpl = 'warm-start bucket sort + leap', per * n
+        note = ('read key, v, idx%s; write the same (%d B per sheet)'
+                % ('' if equal else ', mass', per))
+    elif dom == KC_LEAP:
+        kname, bpl = 'fused force+kick+drift', 44 * n
+        note = 'read key, perm, v, idx; write key, v, idx'
+    else:
+        kname, bpl = 'class %d' % dom, 16 * n
+        note = 'approximate'
+    avg_s = ms * 1e-3 / max(cnt, 1)
+    achieved = bpl / avg_s if avg_s > 0 else 0.
+    traffic, traffic_src = None, None
+    if dom == KC_WARM:
+        traffic, traffic_src = measured_traffic('warm_pipe_n%d_hbm_traffic.md' % n)
+    roofline = {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9,
+                'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': traffic,
+                'traffic_source': traffic_src,
+                'kernel': kname, 'bytes_per_launch': bpl,
+                'avg_launch_us': avg_s * 1e6, 'launches': cnt,
+                'bytes_note': note,
+                # whole-step rate against the full-radix-resort model
+                # (236 / 272 B per sheet-step) and the compulsory state
+                # traffic (40 / 48 B), equal / unequal masses
+                'step_frac_Balg': rate * (236 if equal else 272) / HBM_PEAK,
+                'step_frac_Bmin': rate * (40 if equal else 48) / HBM_PEAK}
+    return {'workload': wl['label'], 'n': n, 'value': rate,
+            'unit': 'sheet-steps/s', 'ms_per_step': 1e3 * elapsed / steps,
+            'steps': steps, 'warmup': warmup, 'elapsed': elapsed,
+            'e2e_value': e2e_rate,
+            'kernel_ms': {str(c): kt[c][0] for c in kt}, 'paths': paths,
+            'warm_config': {'shape': shape, 'bucket_target': target,
+                            'window': window},
+            'roofline': roofline}
 
 
 def main():
@@ -325,7 +472,9 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--workload', choices=sorted(WORKLOADS), default='plain24')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-threads', type=int, default=16)
+    ap.add_argument('--cpu-threads', type=int, default=None,
+                    help='threads for the CPU baseline (default: the CPUs '
+                         'this process may use, capped by OMP_NUM_THREADS)')
     ap.add_argument('--no-ensemble', action='store_true')
     ap.add_argument('--ens-sims', type=int, default=4096)
     ap.add_argument('--ens-steps', type=int, default=10)
@@ -335,6 +484,8 @@ def main():
                     help='only run the calibration copy kernel')
     ap.add_argument('--serve-window', type=float, default=3.)
     ap.add_argument('--serve-rate', type=float, default=200.)
+    ap.add_argument('--no-unequal27', action='store_true')
+    ap.add_argument('--unequal27-steps', type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; --gpus %d is not '
@@ -345,104 +496,25 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
-    import wendy_amd
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    res = bench_plain(args.workload, args.steps, args.warmup, log, e2e=True)
     wl = WORKLOADS[args.workload]
     n, nleap, dt, omega = wl['n'], wl['nleap'], wl['dt'], wl['omega']
-    x, v, m = workload(n, lognormal=wl['lognormal'], vshift=wl['vshift'])
-    equal = wl['lognormal'] is None
-    h = dt / nleap
-    log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    log('bench: n=%d nleap=%d steps=%d warmup=%d' % (n, nleap, args.steps,
-                                                     args.warmup))
-    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.)
-    for _ in range(args.warmup):
-        sim.advance(h, nleap)
-    sim.read()
-    log('bench: warmup done')
-    sim.read()  # synchronise
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.advance(h, nleap)
-    sim.read()  # synchronise
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    rate = n * nleap * args.steps / elapsed
-    log('bench: device-only %.3e sheet-steps/s (%.2f ms/output)'
-        % (rate, 1e3 * elapsed / args.steps))
-    # per-kernel device time: a second timed region of the same work with a
-    # HIP event pair around every launch on the simulation's stream (the
-    # events themselves add a little time between launches, so the headline
-    # rate above is taken without them)
-    kb = max(1, min(args.steps, 20))
-    sim.set_timing(True)
-    for _ in range(kb):
-        sim.advance(h, nleap)
-    sim.read()
-    sim.set_timing(False)
-    kt = {c: sim.kernel_time(c) for c in range(6)}
-    paths = {'warm_steps': sim.path_stats()[0], 'cold_steps': sim.path_stats()[1],
-             'redos': sim.path_stats()[2],
-             'failures_capacity_overflow_stall': list(sim.warm_failures())}
-    log('bench: paths %s' % paths)
-
-    # end-to-end as the nbody generator runs it: each output is de-sorted
-    # and copied to page-locked host arrays while the next interval runs
-    from wendy_amd import _lib as wlib
-    xo = wlib.pinned_empty(n)
-    vo = wlib.pinned_empty(n)
-    ne = max(1, min(10, args.steps))
-    sim.read()
-    sim.advance(h, nleap)
-    sim.read_and_advance(xo, vo, h, nleap)  # the pipeline is full from here
-    t0 = time.perf_counter()
-    for _ in range(ne):
-        sim.read_and_advance(xo, vo, h, nleap)
-    e2e = n * nleap * ne / (time.perf_counter() - t0)
-    log('bench: end-to-end %.3e sheet-steps/s' % e2e)
-    sim.close()
-
-    # dominant kernel and its algorithmic bytes per launch
-    busy = {c: kt[c][0] for c in kt}
-    dom = max(busy, key=busy.get)
-    ms, cnt = kt[dom]
-    if dom == KC_SORT_PASS:
-        name, bpl = 'radix scatter pass', 24 * n
-        note = 'read key 8 B + payload 4 B, write key 8 B + payload 4 B per sheet'
-    elif dom == KC_WARM:
-        per = 40 if equal else 56
-        name, bpl = 'warm-start bucket sort + leap', per * n
-        note = ('read key, v, idx%s; write the same (%d B per sheet)'
-                % ('' if equal else ', mass', per))
-    elif dom == KC_LEAP:
-        name, bpl = 'fused force+kick+drift', 44 * n
-        note = 'read key, perm, v, idx; write key, v, idx'
-    else:
-        name, bpl = 'class %d' % dom, 16 * n
-        note = 'approximate'
-    avg_s = ms * 1e-3 / max(cnt, 1)
-    achieved = bpl / avg_s if avg_s > 0 else 0.
-    traffic, traffic_src = None, None
-    if dom == KC_WARM and args.workload == 'plain24':
-        traffic, traffic_src = measured_traffic('warm_pipe_n16777216_hbm_traffic.md')
-    roofline = {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9,
-                'unit': 'GB/s', 'frac': achieved / HBM_PEAK, 'traffic': traffic,
-                'traffic_source': traffic_src,
-                'kernel': name, 'bytes_per_launch': bpl,
-                'avg_launch_us': avg_s * 1e6, 'launches': cnt,
-                'bytes_note': note,
-                # whole-step rate against the full-radix-resort model
-                # (236 / 272 B per sheet-step) and the compulsory state
-                # traffic (40 / 48 B), equal / unequal masses
-                'step_frac_Balg': rate * (236 if equal else 272) / HBM_PEAK,
-                'step_frac_Bmin': rate * (40 if equal else 48) / HBM_PEAK}
-
+    rate, elapsed = res['value'], res['elapsed']
     main_line = args.workload == 'plain24'
     ens = None if args.no_ensemble or not main_line else bench_ensemble(args, log)
     srv = None if args.no_serve or not main_line else bench_serve(args, log)
 
+    # the large unequal-mass disk (N = 2^27), time-bounded, beside the line
+    big = None
+    if main_line and not args.no_unequal27:
+        big = bench_plain('unequal27', args.unequal27_steps, 2, log, e2e=False)
+        big.pop('elapsed', None)
+
     cpu = None
     if not args.no_cpu_baseline and main_line:
         log('bench: cpu baseline ...')
+        x, v, m = workload(n, lognormal=wl['lognormal'], vshift=wl['vshift'])
         cpu = cpu_baseline(x, v, m, dt, nleap, omega, args.cpu_threads)
         if cpu:
             log('bench: cpu baseline %.3e sheet-steps/s' % cpu['value'])
@@ -455,11 +527,13 @@ def main():
         'data': 'synthetic',
         'config': {'workload': wl['label'], 'n': n, 'nleap': nleap, 'dt': dt,
                    'omega': omega},
-        'e2e_value': e2e,
-        'kernel_ms': {str(c): kt[c][0] for c in kt},
-        'paths': paths,
-        'roofline': roofline,
+        'e2e_value': res['e2e_value'],
+        'kernel_ms': res['kernel_ms'],
+        'paths': res['paths'],
+        'warm_config': res['warm_config'],
+        'roofline': res['roofline'],
         'cpu_baseline': cpu,
+        'unequal27': big,
         'ensemble': ens,
         'serving': srv,
     }

@@ -106,6 +106,13 @@ int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out);
 int wdy_read_and_advance(wdy_sim *sim, double *x_out, double *v_out, double h,
                          int nleap, double *seconds_out);
 
+/* Benchmark calibration: `launches` launches of a plain device copy (read
+ * `bytes`, write `bytes`, 16 B per thread per iteration) between two freshly
+ * allocated buffers; *ms_out (may be NULL) receives the mean milliseconds per
+ * launch.  Its HBM counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) scale those
+ * of the kernels above to bytes.  No reference counterpart. */
+int wdy_copy_probe(int64_t bytes, int launches, double *ms_out);
+
 /* Page-locked host memory for output buffers (hipHostMalloc). */
 int wdy_host_alloc(int64_t bytes, void **out);
 int wdy_host_free(void *ptr);
@@ -286,7 +293,9 @@ int wdy_pool_add(wdy_pool *pool, int64_t n, const double *x, const double *v,
 int wdy_pool_step(wdy_pool *pool, int count, const int *slots, const double *h,
                   const int *nleap);
 /* De-sorted x, v of one slot (either may be NULL); blocks until every step
- * enqueued before it is done. */
+ * enqueued before it is done.  WDY_EVERIFY when this slot's simulation
+ * failed its on-device check (each slot has its own error word, cleared
+ * when the slot is next admitted to). */
 int wdy_pool_read(wdy_pool *pool, int slot, double *x_out, double *v_out);
 /* Free a slot and its arena range (reused only after the steps enqueued so
  * far have finished). */

@@ -15,7 +15,8 @@ import numpy
 from numpy.ctypeslib import ndpointer
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'libwendy_amd.so')
+# WENDY_AMD_LIB names another build of the library (e.g. to compare builds)
+LIB_PATH = os.environ.get('WENDY_AMD_LIB') or os.path.join(HERE, 'libwendy_amd.so')
 
 WDY_OK = 0
 WDY_EINVAL = -1
@@ -38,6 +39,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
             'wdy_set_ext_force', 'wdy_set_ext_force_host', 'wdy_time', 'wdy_warm_failures', 'wdy_warm_config',
             'wdy_read_and_advance', 'wdy_host_alloc', 'wdy_host_free',
+            'wdy_copy_probe',
             'wdy_exact_create', 'wdy_exact_advance', 'wdy_exact_reset',
             'wdy_exact_read', 'wdy_exact_destroy',
             'wdy_batch_create', 'wdy_batch_advance', 'wdy_batch_read',
@@ -130,6 +132,8 @@ def _declare(lib):
     lib.wdy_read_and_advance.restype = ctypes.c_int
     lib.wdy_read_and_advance.argtypes = [S, S, S, ctypes.c_double, ctypes.c_int,
                                          _c_double_p]
+    lib.wdy_copy_probe.restype = ctypes.c_int
+    lib.wdy_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int, _c_double_p]
     lib.wdy_host_alloc.restype = ctypes.c_int
     lib.wdy_host_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(S)]
     lib.wdy_host_free.restype = ctypes.c_int

@@ -537,6 +537,55 @@ int wdy_read_and_advance(wdy_sim *sim, double *x_out, double *v_out, double h,
   return rc;
 }
 
+// Plain streaming copy: each thread moves 16 B per iteration of a grid-
+// stride loop (dwordx4 loads and stores).
+__global__ __launch_bounds__(256) void k_copy_probe(
+    const ulonglong2 *__restrict__ src, ulonglong2 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+int wdy_copy_probe(int64_t bytes, int launches, double *ms_out) {
+  if (bytes < 16 || bytes % 16 != 0 || launches < 1)
+    return fail(WDY_EINVAL, "wdy_copy_probe: bytes must be a positive multiple "
+                "of 16 and launches >= 1");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(WDY_ENODEV, "wdy_copy_probe: no GPU visible");
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = [&]() -> int {
+    WDY_HIP(hipMalloc(&a, (size_t)bytes));
+    WDY_HIP(hipMalloc(&b, (size_t)bytes));
+    WDY_HIP(hipMemset(a, 1, (size_t)bytes));
+    WDY_HIP(hipEventCreate(&e0));
+    WDY_HIP(hipEventCreate(&e1));
+    int dev = 0, cus = 0;
+    WDY_HIP(hipGetDevice(&dev));
+    WDY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t n = (size_t)bytes / 16;
+    const unsigned grid = (unsigned)std::max(1, cus) * 8u;
+    WDY_HIP(hipDeviceSynchronize());
+    WDY_HIP(hipEventRecord(e0, nullptr));
+    for (int k = 0; k < launches; ++k) {
+      k_copy_probe<<<grid, 256>>>((const ulonglong2 *)a, (ulonglong2 *)b, n);
+      WDY_CHECK_LAUNCH();
+    }
+    WDY_HIP(hipEventRecord(e1, nullptr));
+    WDY_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    WDY_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_out) *ms_out = ms / launches;
+    return WDY_OK;
+  }();
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 int wdy_host_alloc(int64_t bytes, void **out) {
   if (!out || bytes < 0) return fail(WDY_EINVAL, "wdy_host_alloc: bad arguments");
   *out = nullptr;

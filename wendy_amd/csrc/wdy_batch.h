@@ -12,6 +12,8 @@ struct BatchDesc {
   u32 flags;      // 1 equal masses, 2 harmonic term
   int cur;        // current buffer set
   int nleap;      // leapfrog steps this interval (0: skip)
+  u32 eslot;      // error word of this simulation: err[eslot]
+  u32 pad_;
   double h;       // leapfrog step
   double mq, totmass, omega2, fscale, finv;
 };
@@ -26,7 +28,8 @@ struct BatchArrays {
   double *mass[2];
 };
 
-// One workgroup per descriptor; descriptors in d_descs[0..count).
+// One workgroup per descriptor; descriptors in d_descs[0..count).  A
+// simulation that fails its on-device check sets err[its eslot].
 int batch_launch(hipStream_t st, const BatchDesc *d_descs, int count,
                  const BatchArrays &arr, u32 *err);
 

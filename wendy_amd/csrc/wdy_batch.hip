@@ -305,7 +305,7 @@ __global__ __launch_bounds__(BWT) void k_batch_advance(
       sorted = s_flag != 0;
       __syncthreads();
     }
-    if (!sorted && t == 0) atomicOr(err, 1u);  // not expected: bound above
+    if (!sorted && t == 0) atomicOr(err + d.eslot, 1u);  // not expected: bound above
     // force, kick, drift, block by block, into the other buffer set
     const int b = a ^ 1;
     u64 *ko = A.key[b] + d.off;
@@ -356,6 +356,8 @@ void batch_desc_fill(BatchDesc &d, const double *m, u32 n, double totmass,
   d.flags = (eq ? 1u : 0u) | (omega2 >= 0.0 ? 2u : 0u);
   d.cur = 0;
   d.nleap = 0;
+  d.eslot = 0;
+  d.pad_ = 0;
   d.h = 0.0;
   d.mq = m[0];
   d.totmass = totmass;

@@ -25,8 +25,10 @@ namespace wdy {
 namespace {
 
 __global__ void k_pool_init(u32 n, const double *__restrict__ x,
-                            u64 *__restrict__ key, i32 *__restrict__ idx) {
+                            u64 *__restrict__ key, i32 *__restrict__ idx,
+                            u32 *__restrict__ err) {
   const u32 i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *err = 0;  // the slot's error word, for its new occupant
   if (i >= n) return;
   key[i] = encode_pos(x[i]);
   idx[i] = (i32)i;  // sheet index within its simulation
@@ -171,8 +173,9 @@ int wdy_pool_create(int64_t capacity_sheets, int max_sims, int64_t max_n,
     WDY_HIP(hipMalloc(&p->stage_x, (size_t)max_n * 8));
     WDY_HIP(hipMalloc(&p->read_x, (size_t)max_n * 8));
     WDY_HIP(hipMalloc(&p->read_v, (size_t)max_n * 8));
-    WDY_HIP(hipMalloc(&p->err, 8));
-    WDY_HIP(hipMemsetAsync(p->err, 0, 8, p->stream));
+    // one error word per slot, cleared by each admission's upload
+    WDY_HIP(hipMalloc(&p->err, (size_t)max_sims * 4));
+    WDY_HIP(hipMemsetAsync(p->err, 0, (size_t)max_sims * 4, p->stream));
     WDY_HIP(hipStreamSynchronize(p->stream));
     p->slots.resize((size_t)max_sims);
     p->free_ranges[0] = p->cap;
@@ -215,13 +218,15 @@ int wdy_pool_add(wdy_pool *p, int64_t n, const double *x, const double *v,
   Slot &s = p->slots[(size_t)slot];
   batch_desc_fill(s.d, m, (u32)n, totmass, omega2);
   s.d.off = off;
+  s.d.eslot = (u32)slot;
   const size_t bytes = (size_t)n * 8;
   int rc = [&]() -> int {
     WDY_HIP(hipMemcpyAsync(p->stage_x, x, bytes, hipMemcpyHostToDevice, p->up));
     WDY_HIP(hipMemcpyAsync(p->arr.vel[0] + off, v, bytes, hipMemcpyHostToDevice, p->up));
     WDY_HIP(hipMemcpyAsync(p->arr.mass[0] + off, m, bytes, hipMemcpyHostToDevice, p->up));
     k_pool_init<<<(unsigned)((n + 255) / 256), 256, 0, p->up>>>(
-        (u32)n, p->stage_x, p->arr.key[0] + off, p->arr.idx[0] + off);
+        (u32)n, p->stage_x, p->arr.key[0] + off, p->arr.idx[0] + off,
+        p->err + slot);
     WDY_CHECK_LAUNCH();
     WDY_HIP(hipEventRecord(p->up_done, p->up));
     return WDY_OK;
@@ -298,11 +303,11 @@ int wdy_pool_read(wdy_pool *p, int slot, double *x_out, double *v_out) {
     WDY_HIP(hipMemcpyAsync(v_out, p->read_v, (size_t)d.n * 8, hipMemcpyDeviceToHost,
                            p->stream));
   u32 err = 0;
-  WDY_HIP(hipMemcpyAsync(&err, p->err, 4, hipMemcpyDeviceToHost, p->stream));
+  WDY_HIP(hipMemcpyAsync(&err, p->err + slot, 4, hipMemcpyDeviceToHost, p->stream));
   WDY_HIP(hipStreamSynchronize(p->stream));
   if (err)
-    return fail(WDY_EVERIFY, "wdy_pool_read: a simulation did not sort within its "
-                "bound");
+    return fail(WDY_EVERIFY, "wdy_pool_read: slot %d did not sort within its "
+                "bound", slot);
   return WDY_OK;
 }
 

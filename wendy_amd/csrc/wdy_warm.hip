@@ -199,12 +199,16 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const u32 e = i * WT + t;
-        if (e < c) {
-          const bool seg = e < nseg;
-          k[i] = (seg ? A.kin : A.rd.okey)[pos[i]];
-          v[i] = (seg ? A.vin : A.rd.ov)[pos[i]];
-          id[i] = (seg ? A.iin : A.rd.oid)[pos[i]];
-          if (!EQ) m[i] = (seg ? A.min : A.rd.om)[pos[i]];
+        if (e < nseg) {  // (separate branches: see k_warm_pipe's data_issue)
+          k[i] = A.kin[pos[i]];
+          v[i] = A.vin[pos[i]];
+          id[i] = A.iin[pos[i]];
+          if (!EQ) m[i] = A.min[pos[i]];
+        } else if (e < c) {
+          k[i] = A.rd.okey[pos[i]];
+          v[i] = A.rd.ov[pos[i]];
+          id[i] = A.rd.oid[pos[i]];
+          if (!EQ) m[i] = A.rd.om[pos[i]];
         }
       }
 #pragma unroll
@@ -573,7 +577,12 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     return short_excl_u32_inplace<NWAVE>(s_segpre, S, s_red, &s_tot);
   };
 
-  // data registers of the bucket being prefetched
+  // data registers of the bucket being prefetched.  Segment sheets and far
+  // movers load in separate branches: a per-lane choice between two of the
+  // argument's pointers would make the compiler fetch the pointer itself
+  // from the argument block before each load, and the wait for it (vector
+  // loads retire in order) would serialise every earlier load of the
+  // prefetch behind it.
   u64 dk[PER];
   double dv[PER], dm[PER];
   i32 di[PER];
@@ -581,14 +590,18 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const u32 e = i * WT + t;
-      if (e < c) {
-        const bool seg = e < nseg;
-        const u32 pos = seg ? seg_pos(s_segpre, s_segsrc, S, e)
-                            : s_oidx[e - nseg];
-        dk[i] = (seg ? A.kin : A.rd.okey)[pos];
-        dv[i] = (seg ? A.vin : A.rd.ov)[pos];
-        di[i] = (seg ? A.iin : A.rd.oid)[pos];
-        if (!EQ) dm[i] = (seg ? A.min : A.rd.om)[pos];
+      if (e < nseg) {
+        const u32 pos = seg_pos(s_segpre, s_segsrc, S, e);
+        dk[i] = A.kin[pos];
+        dv[i] = A.vin[pos];
+        di[i] = A.iin[pos];
+        if (!EQ) dm[i] = A.min[pos];
+      } else if (e < c) {
+        const u32 q = s_oidx[e - nseg];
+        dk[i] = A.rd.okey[q];
+        dv[i] = A.rd.ov[q];
+        di[i] = A.rd.oid[q];
+        if (!EQ) dm[i] = A.rd.om[q];
       }
     }
   };

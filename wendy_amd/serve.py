@@ -148,8 +148,11 @@ class Request:
 
 
 class Result:
+    """``where``: 'pool' or 'sim'; ``paths`` (large-path requests): the
+    Sim's (warm steps, radix steps, redone intervals) and its warm-failure
+    counts (capacity, overflow, stall)."""
     __slots__ = ('id', 'x', 'v', 'arrival', 'admitted', 'completed', 'n', 'steps',
-                 'where')
+                 'where', 'paths')
 
     @property
     def latency(self):
@@ -171,19 +174,22 @@ class _Big:
         x = numpy.empty(n)
         v = numpy.empty(n)
         self.sim.read(x, v)
+        self.paths = self.sim.path_stats() + self.sim.warm_failures()
         self.sim.close()
         return x, v
 
 
 def _run_big(big, req, tick):
     """All output intervals of one large-path simulation (a worker thread;
-    the library releases the GIL while the device works)."""
+    the library releases the GIL while the device works); returns (x, v,
+    paths)."""
     left = req.steps
     while left:
         k = min(tick, left)
         big.advance(req.h, k)
         left -= k
-    return big.finish(req.n)
+    x, v = big.finish(req.n)
+    return x, v, getattr(big, 'paths', None)
 
 
 class Server:
@@ -252,6 +258,7 @@ class Server:
             res = Result()
             res.id, res.n, res.steps, res.arrival = r.id, r.n, r.steps, r.arrival
             res.where = where
+            res.paths = None
             res.admitted = now()
             return res
 
@@ -292,7 +299,7 @@ class Server:
             nonlocal big_bytes, work
             for f in done:
                 res = big.pop(f)
-                res.x, res.v = f.result()
+                res.x, res.v, res.paths = f.result()
                 res.completed = now()
                 results[res.id] = res
                 work += res.n * res.steps
