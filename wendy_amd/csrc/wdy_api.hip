@@ -135,6 +135,8 @@ static void free_sim(wdy_sim *s) {
   radix_free(&s->radix);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ev_rerr) (void)hipEventDestroy(s->ev_rerr);
+  if (s->h_rerr) (void)hipHostFree(s->h_rerr);
   if (s->ev_out) (void)hipEventDestroy(s->ev_out);
   if (s->ev_copied) (void)hipEventDestroy(s->ev_copied);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
@@ -219,6 +221,9 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     WDY_HIP(hipEventCreate(&s->ev1));
     WDY_HIP(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
     WDY_HIP(hipEventCreateWithFlags(&s->ev_copied, hipEventDisableTiming));
+    WDY_HIP(hipEventCreateWithFlags(&s->ev_rerr, hipEventDisableTiming));
+    WDY_HIP(hipHostMalloc((void **)&s->h_rerr, 8, hipHostMallocDefault));
+    *s->h_rerr = 0;
     s->timer = timer_create();
     if (!s->timer) return fail(WDY_ENOMEM, "wdy_create: timer allocation failed");
     s->n = n;
@@ -273,7 +278,12 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
       if (r) return r;
       s->disp_rate = displacement_rate(n, x_slot, v_id, id_slot);
     }
+    WDY_HIP(hipMemcpyAsync(s->h_rerr, radix_err_word(&s->radix), 4,
+                           hipMemcpyDeviceToHost, s->stream));
     WDY_HIP(hipStreamSynchronize(s->stream));
+    if (*s->h_rerr)
+      return fail(WDY_ERUNTIME, "wdy_create: a look-back wait of the initial "
+                  "sort gave up");
     return WDY_OK;
   }();
   if (rc) {
@@ -396,17 +406,42 @@ static int cold_interval(wdy_sim *s, double h, int nleap, double *acc_last,
 // far movers, a bounded wait exceeded), redoes the interval from its
 // untouched start on the radix path.  Every entry point that reads the
 // state or extends it calls this first.
+// Copies the radix sort's error word to the host behind the work enqueued
+// so far (checked by the next settle).
+static int radix_check_enqueue(wdy_sim *s) {
+  WDY_HIP(hipMemcpyAsync(s->h_rerr, radix_err_word(&s->radix), 4,
+                         hipMemcpyDeviceToHost, s->stream));
+  WDY_HIP(hipEventRecord(s->ev_rerr, s->stream));
+  s->rerr_pending = true;
+  return WDY_OK;
+}
+
 static int settle(wdy_sim *s) {
-  if (!s->warm) return WDY_OK;
+  if (s->broken)
+    return fail(WDY_ERUNTIME, "a radix sort of this simulation gave up a "
+                "bounded wait; its state is lost");
   WDY_HIP(hipSetDevice(s->device));
-  bool failed = false;
-  double h = 0.0;
-  int nleap = 0;
-  if (int rc = warm_settle(s, &failed, &h, &nleap)) return rc;
-  if (!failed) return WDY_OK;
-  s->redos += 1;
-  if (int rc = cold_interval(s, h, nleap, nullptr, nullptr)) return rc;
-  WDY_HIP(hipEventRecord(s->ev1, s->stream));
+  if (s->warm) {
+    bool failed = false;
+    double h = 0.0;
+    int nleap = 0;
+    if (int rc = warm_settle(s, &failed, &h, &nleap)) return rc;
+    if (failed) {
+      s->redos += 1;
+      if (int rc = cold_interval(s, h, nleap, nullptr, nullptr)) return rc;
+      WDY_HIP(hipEventRecord(s->ev1, s->stream));
+      if (int rc = radix_check_enqueue(s)) return rc;
+    }
+  }
+  if (s->rerr_pending) {
+    WDY_HIP(hipEventSynchronize(s->ev_rerr));
+    s->rerr_pending = false;
+    if (*s->h_rerr) {
+      s->broken = true;
+      return fail(WDY_ERUNTIME, "a look-back wait of a radix sort gave up; "
+                  "the simulation's state is lost");
+    }
+  }
   return WDY_OK;
 }
 
@@ -417,6 +452,7 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
   WDY_HIP(hipEventRecord(s->ev0, s->stream));
   if (s->host_force) {
     if (int rc = advance_host_force(s, h, nleap, acc_last, cum_last)) return rc;
+    if (int rc = radix_check_enqueue(s)) return rc;
   } else if (s->small) {
     int rc = launch_small_advance(s, h, nleap, acc_last, cum_last);
     if (rc) return rc;
@@ -429,6 +465,7 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
     if (int rc = warm_interval(s, h, nleap)) return rc;
   } else {
     if (int rc = cold_interval(s, h, nleap, acc_last, cum_last)) return rc;
+    if (int rc = radix_check_enqueue(s)) return rc;
   }
   WDY_HIP(hipEventRecord(s->ev1, s->stream));
   return WDY_OK;
@@ -858,6 +895,7 @@ void _wendy_nbody_approx_onestep(int N, struct array_w_index *xi, double *x,
     // small mode writes the last acceleration per sheet, large mode per slot
     int r = advance_impl(s, dt, nleap, s->small ? acc_o : acc, cum_o);
     if (r) return r;
+    if ((r = settle(s))) return r;
     r = launch_desort(s, s->out_x, s->out_v, s->small ? nullptr : acc, acc_o,
                       nullptr);
     if (r) return r;

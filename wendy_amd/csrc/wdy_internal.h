@@ -130,16 +130,37 @@ int timer_collect(Timer *t, int kc, double *ms_total, int64_t *count);
 // Radix sort of (uint64 key, uint32 payload) pairs: LSD, 8 passes of 8 bits,
 // one all-digit histogram pass plus one onesweep scatter pass per digit.
 // --------------------------------------------------------------------------
+constexpr u32 RADIX_SPIN_LIMIT = 1u << 20;  // probes of one look-back wait
 struct RadixWork {
   u32 n = 0;
   u32 ntiles = 0;
   u32 *hist = nullptr;       // [8][256] digit counts, then exclusive offsets
-  u32 *counters = nullptr;   // [8] tile tickets, one per pass
+  u32 *counters = nullptr;   // [64]: [0..8) tile tickets, one per pass, and
+                             // the error word (radix_err_word)
   u64 *status = nullptr;     // [ntiles][256] lookback words
   u32 epoch = 0;             // sequence number of the next pass
+  // clear the error word at the start of every sort (else it accumulates
+  // until its owner clears it)
+  bool clear_err = false;
 };
 
+// sizes of a sort's scratch for n keys (for callers that provide it)
+u32 radix_tiles(u32 n);
+size_t radix_hist_bytes();
+size_t radix_counter_bytes();
+size_t radix_status_bytes(u32 n);  // look-back words: zero before first use
 int radix_alloc(RadixWork *w, u32 n);
+// Source slot of sorted position i.  A sort whose look-back wait gave up can
+// leave slots of its payload unwritten (arbitrary values); clamping keeps
+// every access that follows inside the arrays until the host reports the
+// failure.  In normal operation perm[i] < n always.
+__device__ __forceinline__ u32 sorted_slot(const u32 *perm, u32 i, u32 n) {
+  const u32 s = perm[i];
+  return s < n ? s : n - 1;
+}
+// Device word set nonzero when a look-back wait of a sort gave up (its
+// results are then wrong; every kernel still finishes).
+u32 *radix_err_word(RadixWork *w);
 void radix_free(RadixWork *w);
 // Sorts keys[0..n) ascending, stably; keys ping-pong between k0 and k1 and
 // end in k0 (8 passes).  The payload ends in p0: if identity_payload, the
@@ -209,6 +230,13 @@ struct wdy_sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   wdy::Timer *timer = nullptr;
   bool timing = false;
+  // the radix sort's error word (a bounded look-back wait gave up), copied
+  // to h_rerr behind the steps that sorted and checked by the next call;
+  // once set the state is lost and every later call fails
+  wdy::u32 *h_rerr = nullptr;
+  hipEvent_t ev_rerr = nullptr;
+  bool rerr_pending = false;
+  bool broken = false;
 };
 
 namespace wdy {
@@ -233,6 +261,18 @@ int launch_build_state(wdy_sim *s, const double *x_slot, const i32 *id_slot,
 int energy_core(hipStream_t st, u32 n, const double *x, const double *v,
                 const double *m, double mq, double mscale, double twopiG,
                 double omega2, double *e_out);
+// The same, enqueued only: scratch is carved from ws (at least
+// energy_scratch_bytes(n, m != null) bytes, 256-byte aligned); status holds
+// the sort's look-back words (radix_status_bytes(n)), zero or left by earlier
+// sorts with other epochs; *epoch is the next sort epoch and is advanced.
+// The result goes to e_dev (device memory); it is NaN when a look-back wait
+// of the sort gave up, which also sets the device word *err_word (if
+// err_word is given).  No allocation, no host synchronisation.
+size_t energy_scratch_bytes(u32 n, bool masses);
+int energy_enqueue(hipStream_t st, u32 n, const double *x, const double *v,
+                   const double *m, double mq, double mscale, double twopiG,
+                   double omega2, double *e_dev, u64 *status, void *ws,
+                   u32 *epoch, const u32 **err_word);
 // small mode (wdy_small.hip)
 u32 small_max_n(int device);
 int launch_small_advance(wdy_sim *s, double h, int nleap, double *acc_out,

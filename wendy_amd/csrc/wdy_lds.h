@@ -76,24 +76,28 @@ __device__ __forceinline__ bool pair_lt(u64 ka, i32 ia, u64 kb, i32 ib) {
 }
 
 // One tile shape: threads per workgroup and the most sheets a bucket holds.
-// WT threads, CAP sheets per bucket, NQ position bins for the bucket sort
-// (also the per-wave segment counters of the warm step's partition).
-// OCC_EQ: waves per SIMD the equal-mass pipelined kernel is compiled for
-// (its VGPR budget: 512 / OCC_EQ); 1 leaves the register count to the
-// compiler.
-template <int WT_, int CAP_, int NQ_ = CAP_, int OCC_EQ_ = 1>
+// WT threads, CAP sheets per bucket, NQ position bins for the bucket sort.
+// PACK: two 16-bit bin counters per 32-bit word (counts and offsets never
+// exceed CAP < 2^16), so twice the bins fit the same LDS.  The bin words
+// (NQW of them) double as the per-wave segment counters of the warm step's
+// partition.  OCC_EQ: waves per SIMD the equal-mass pipelined kernel is
+// compiled for (its VGPR budget: 512 / OCC_EQ); 1 leaves the register
+// count to the compiler.
+template <int WT_, int CAP_, int NQ_ = CAP_, int OCC_EQ_ = 1, bool PACK_ = false>
 struct Shape {
   static constexpr int WT = WT_, CAP = CAP_, NQ = NQ_, PER = CAP_ / WT_;
   static constexpr int NWAVE = WT_ / 64, OCC_EQ = OCC_EQ_;
-  static_assert(CAP_ % WT_ == 0 && NQ_ % WT_ == 0 && WT_ % 64 == 0 &&
-                    CAP_ <= 65535, "shape");
+  static constexpr bool PACK = PACK_;
+  static constexpr int NQW = PACK_ ? NQ_ / 2 : NQ_;
+  static_assert(CAP_ % WT_ == 0 && NQW % WT_ == 0 && NQ_ % WT_ == 0 &&
+                    WT_ % 64 == 0 && CAP_ <= 65535, "shape");
 };
 
 template <class SH, bool EQ>
 struct Lds {
   static constexpr int CAP = SH::CAP;
   static constexpr size_t BYTES =
-      (size_t)CAP * (EQ ? 22 : 30) + (size_t)SH::NQ * 4;
+      (size_t)CAP * (EQ ? 22 : 30) + (size_t)SH::NQW * 4;
   static_assert(BYTES % 8 == 0, "what follows the arrays stays 8-byte aligned");
   u64 *key;
   double *v;
@@ -140,7 +144,7 @@ __device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
     s_mm[2 * (t >> 6) + 1] = kmax;
   }
 #pragma unroll
-  for (int i = 0; i < NQ / WT; ++i) s.bins[i * WT + t] = 0;
+  for (int i = 0; i < SH::NQW / WT; ++i) s.bins[i * WT + t] = 0;
   __syncthreads();
   kmin = ~0ull;
   kmax = 0ull;
@@ -176,57 +180,81 @@ __device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
         b = (u32)((k - kmin) >> kshift);
       }
       q[i] = b;
-      atomicAdd(&s.bins[b], 1u);
+      if (SH::PACK)
+        atomicAdd(&s.bins[b >> 1], 1u << ((b & 1u) * 16u));
+      else
+        atomicAdd(&s.bins[b], 1u);
     }
   }
   __syncthreads();
   if (prof && t == 0) prof[6] = wall_clock64();
   {
-    constexpr int BPT = NQ / WT;  // bins t*BPT .. t*BPT + BPT - 1
-    u32 loc[BPT];
+    constexpr int WPT = SH::NQW / WT;  // bin words t*WPT .. t*WPT + WPT - 1
+    u32 loc[WPT];
     u64 sum = 0;
 #pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-      loc[i] = s.bins[t * BPT + i];
-      sum += loc[i];
+    for (int i = 0; i < WPT; ++i) {
+      loc[i] = s.bins[t * WPT + i];
+      sum += SH::PACK ? (loc[i] & 0xffffu) + (loc[i] >> 16) : loc[i];
     }
     u64 run = block_excl_u64<NWAVE>(sum, s_red, nullptr);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-      s.bins[t * BPT + i] = (u32)run;
-      run += loc[i];
+    for (int i = 0; i < WPT; ++i) {
+      if (SH::PACK) {  // exclusive offsets of both bins of the word
+        const u32 lo = (u32)run, hi = lo + (loc[i] & 0xffffu);
+        s.bins[t * WPT + i] = lo | (hi << 16);
+        run += (loc[i] & 0xffffu) + (loc[i] >> 16);
+      } else {
+        s.bins[t * WPT + i] = (u32)run;
+        run += loc[i];
+      }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const u32 e = i * WT + t;
-    if (e < c) s.ord[atomicAdd(&s.bins[q[i]], 1u)] = (unsigned short)e;
+    if (e < c) {
+      u32 slot;
+      if (SH::PACK) {
+        const u32 sh = (q[i] & 1u) * 16u;
+        slot = (atomicAdd(&s.bins[q[i] >> 1], 1u << sh) >> sh) & 0xffffu;
+      } else {
+        slot = atomicAdd(&s.bins[q[i]], 1u);
+      }
+      s.ord[slot] = (unsigned short)e;
+    }
   }
   __syncthreads();
   if (prof && t == 0) prof[7] = wall_clock64();
-  // bins[b] is now the end of bin b.  Rank each sheet inside its bin by
+  // bin b now ends at bin_end(b).  Rank each sheet inside its bin by
   // counting the bin's smaller (key, id) pairs -- every sheet in parallel, a
-  // bin of k sheets costs k comparisons per sheet.
+  // bin of k sheets costs k comparisons per sheet (none when it is alone).
+  auto bin_end = [&](u32 b) -> u32 {
+    return SH::PACK ? (s.bins[b >> 1] >> ((b & 1u) * 16u)) & 0xffffu : s.bins[b];
+  };
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const u32 e = i * WT + t;
     rk[i] = 0;
     if (e < c) {
-      const u32 lo = q[i] == 0 ? 0u : s.bins[q[i] - 1], hi = s.bins[q[i]];
-      const u64 k = s.key[e];
-      const i32 id = s.id[e];
-      u32 r = lo, a = lo;
-      for (; a + 2 <= hi; a += 2) {  // two at a time: the LDS loads overlap
-        const unsigned short f0 = s.ord[a], f1 = s.ord[a + 1];
-        const u64 k0 = s.key[f0], k1 = s.key[f1];
-        const i32 i0 = s.id[f0], i1 = s.id[f1];
-        r += (pair_lt(k0, i0, k, id) ? 1u : 0u) + (pair_lt(k1, i1, k, id) ? 1u : 0u);
-      }
-      if (a < hi) {
-        const unsigned short f = s.ord[a];
-        r += pair_lt(s.key[f], s.id[f], k, id) ? 1u : 0u;
+      const u32 lo = q[i] == 0 ? 0u : bin_end(q[i] - 1), hi = bin_end(q[i]);
+      u32 r = lo;
+      if (hi - lo > 1) {
+        const u64 k = s.key[e];
+        const i32 id = s.id[e];
+        u32 a = lo;
+        for (; a + 2 <= hi; a += 2) {  // two at a time: the LDS loads overlap
+          const unsigned short f0 = s.ord[a], f1 = s.ord[a + 1];
+          const u64 k0 = s.key[f0], k1 = s.key[f1];
+          const i32 i0 = s.id[f0], i1 = s.id[f1];
+          r += (pair_lt(k0, i0, k, id) ? 1u : 0u) + (pair_lt(k1, i1, k, id) ? 1u : 0u);
+        }
+        if (a < hi) {
+          const unsigned short f = s.ord[a];
+          r += pair_lt(s.key[f], s.id[f], k, id) ? 1u : 0u;
+        }
       }
       rk[i] = r;
     }

@@ -16,6 +16,8 @@
 // (1 = tile aggregate, 2 = inclusive prefix), [31:0] count.  The epoch makes
 // stale words from earlier passes invisible, so the status array is never
 // cleared between passes.
+#include <cstdlib>
+
 #include "wdy_internal.h"
 
 namespace wdy {
@@ -54,9 +56,13 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32 *s_w) {
   return wp + x - v;
 }
 
-__global__ void k_sort_init(u32 *hist, u32 *counters) {
+// Clears the histograms and the pass tickets; with clear_err also the error
+// word counters[ERR_WORD] of a sort whose caller checks it per sort.
+constexpr u32 ERR_WORD = 32;
+__global__ void k_sort_init(u32 *hist, u32 *counters, u32 clear_err) {
   for (u32 i = threadIdx.x; i < PASSES * RADIX; i += blockDim.x) hist[i] = 0;
   if (threadIdx.x < PASSES) counters[threadIdx.x] = 0;
+  if (clear_err && threadIdx.x == 0) counters[ERR_WORD] = 0;
 }
 
 // Counts all 8 digits of every key.  Grid-stride over tiles.
@@ -101,7 +107,8 @@ template <bool IDENT>
 __global__ __launch_bounds__(THREADS) void k_scatter(
     const u64 *__restrict__ kin, const u32 *__restrict__ pin,
     u64 *__restrict__ kout, u32 *__restrict__ pout, u32 n, u32 shift,
-    const u32 *__restrict__ offs, u64 *status, u32 *counter, u32 epoch) {
+    const u32 *__restrict__ offs, u64 *status, u32 *counter, u32 epoch,
+    u32 spin_limit, u32 *err) {
   __shared__ u32 s_tile;
   __shared__ u32 s_cnt[WAVES][RADIX];
   __shared__ u32 s_dbase[RADIX];
@@ -174,10 +181,20 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
     } else {
       atomic_store_agent(mine, ep | (1ull << 32) | tot);
       int64_t pt = (int64_t)tile - 1;
-      while (true) {
+      // every earlier tile publishes without waiting on a later one, so the
+      // wait ends; the bound turns a broken invariant into a reported error
+      // (a limit of 0 gives up at once).  A tile that gave up still
+      // publishes, so that every later tile and the kernel finish.
+      bool gave_up = spin_limit == 0;
+      u32 spins = 0;
+      while (!gave_up) {
         const u64 s = atomic_load_agent(status + (u64)pt * RADIX + t);
         const u32 flag = (u32)(s >> 32) & 3u;
         if ((u32)(s >> 34) != (epoch & 0x3FFFFFFFu) || flag == 0) {
+          if (++spins > spin_limit) {
+            gave_up = true;
+            break;
+          }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
@@ -185,6 +202,7 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
         if (flag == 2) break;
         --pt;
       }
+      if (gave_up) atomicOr(err, 1u);
       atomic_store_agent(mine, ep | (2ull << 32) | (excl + tot));
     }
     s_gbase[t] = offs[t] + excl - dex;
@@ -206,12 +224,26 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
     const u64 kk = s_key[i];
     const u32 d = (u32)(kk >> shift) & 255u;
     const u32 g = s_gbase[d] + i;
-    kout[g] = kk;
-    pout[g] = s_pay[i];
+    // always true unless a look-back wait of this or an earlier pass gave
+    // up: its tile's offsets, and the digit counts of the keys a later pass
+    // reads, are then wrong, and a write could land past the array
+    if (g < n) {
+      kout[g] = kk;
+      pout[g] = s_pay[i];
+    }
   }
 }
 
 }  // namespace
+
+u32 *radix_err_word(RadixWork *w) { return w->counters + ERR_WORD; }
+
+u32 radix_tiles(u32 n) { return n == 0 ? 1u : (n + TILE - 1) / TILE; }
+size_t radix_hist_bytes() { return (size_t)PASSES * RADIX * sizeof(u32); }
+size_t radix_counter_bytes() { return 64 * sizeof(u32); }
+size_t radix_status_bytes(u32 n) {
+  return (size_t)radix_tiles(n) * RADIX * sizeof(u64);
+}
 
 int radix_alloc(RadixWork *w, u32 n) {
   w->n = n;
@@ -219,6 +251,7 @@ int radix_alloc(RadixWork *w, u32 n) {
   if (w->ntiles == 0) w->ntiles = 1;
   WDY_HIP(hipMalloc(&w->hist, PASSES * RADIX * sizeof(u32)));
   WDY_HIP(hipMalloc(&w->counters, 64 * sizeof(u32)));
+  WDY_HIP(hipMemset(w->counters, 0, 64 * sizeof(u32)));
   WDY_HIP(hipMalloc(&w->status, (size_t)w->ntiles * RADIX * sizeof(u64)));
   WDY_HIP(hipMemset(w->status, 0, (size_t)w->ntiles * RADIX * sizeof(u64)));
   w->epoch = 1;
@@ -238,7 +271,12 @@ int radix_sort_pairs(RadixWork *w, hipStream_t s, Timer *tm, u64 *k0, u64 *k1,
                      u32 *p0, u32 *p1, bool identity_payload) {
   const u32 n = w->n;
   if (n == 0) return WDY_OK;
-  k_sort_init<<<1, THREADS, 0, s>>>(w->hist, w->counters);
+  // bound of each look-back wait, in probes (WDY_RADIX_SPIN_LIMIT overrides
+  // it; 0 makes every wait give up at once, so that tests can reach the
+  // failure path with every kernel still finishing)
+  u32 spin_limit = RADIX_SPIN_LIMIT;
+  if (const char *e = getenv("WDY_RADIX_SPIN_LIMIT")) spin_limit = (u32)strtoul(e, nullptr, 10);
+  k_sort_init<<<1, THREADS, 0, s>>>(w->hist, w->counters, w->clear_err ? 1u : 0u);
   WDY_CHECK_LAUNCH();
   const u32 hist_blocks = w->ntiles < 2048 ? w->ntiles : 2048;
   if (tm) timer_begin(tm, s, KC_SORT_HIST);
@@ -259,11 +297,11 @@ int radix_sort_pairs(RadixWork *w, hipStream_t s, Timer *tm, u64 *k0, u64 *k1,
     if (pass == 0 && identity_payload)
       k_scatter<true><<<w->ntiles, THREADS, 0, s>>>(
           ks, ps, kd, pd, n, 8u * pass, w->hist + pass * RADIX, w->status,
-          w->counters + pass, ep);
+          w->counters + pass, ep, spin_limit, w->counters + ERR_WORD);
     else
       k_scatter<false><<<w->ntiles, THREADS, 0, s>>>(
           ks, ps, kd, pd, n, 8u * pass, w->hist + pass * RADIX, w->status,
-          w->counters + pass, ep);
+          w->counters + pass, ep, spin_limit, w->counters + ERR_WORD);
     if (tm) timer_end(tm, s, KC_SORT_PASS);
     WDY_CHECK_LAUNCH();
   }

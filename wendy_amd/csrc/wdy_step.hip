@@ -13,6 +13,7 @@
 // (wendy/wendy.c:359-360).  Equal-mass runs use C = rank * m, the exact
 // value of the reference's running sum whenever m is a power of two.
 #include <climits>
+#include <cmath>
 
 #include "wdy_internal.h"
 
@@ -53,7 +54,7 @@ __device__ double tied_prefix(const u64 *__restrict__ ks,
   if (EQ) {
     u32 c = 0;
     for (u32 j = s; j < e; ++j)
-      if (j != i && idx_old[perm[j]] < myid) ++c;
+      if (j != i && idx_old[sorted_slot(perm, j, n)] < myid) ++c;
     return (double)(s + c) * mq;
   } else {
     double C = cum[s];
@@ -65,19 +66,20 @@ __device__ double tied_prefix(const u64 *__restrict__ ks,
         i32 best = INT_MAX;
         u32 bj = s;
         for (u32 j = s; j < e; ++j) {
-          const i32 id = idx_old[perm[j]];
+          const i32 id = idx_old[sorted_slot(perm, j, n)];
           if (id > last && id < myid && id < best) {
             best = id;
             bj = j;
           }
         }
         if (best == INT_MAX) break;
-        C = C + mass_old[perm[bj]];
+        C = C + mass_old[sorted_slot(perm, bj, n)];
         last = best;
       }
     } else {
       for (u32 j = s; j < e; ++j)
-        if (idx_old[perm[j]] < myid) C = C + mass_old[perm[j]];
+        if (idx_old[sorted_slot(perm, j, n)] < myid)
+          C = C + mass_old[sorted_slot(perm, j, n)];
     }
     return C;
   }
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(BLOCK) void k_leap(
   const u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const u64 k = ks[i];
-  const u32 src = perm[i];
+  const u32 src = sorted_slot(perm, i, n);
   double x = decode_pos(k);
   double v = vel_old[src];
   const i32 id = idx_old[src];
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_state(
     double *__restrict__ mass) {
   const u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  const i32 id = id_slot[perm[i]];
+  const i32 id = id_slot[sorted_slot(perm, i, n)];
   key[i] = ks[i];
   idx[i] = id;
   if (vel) vel[i] = v_id[id];
@@ -150,8 +152,8 @@ __global__ __launch_bounds__(BLOCK) void k_build_state(
 // ---- exclusive FP64 mass prefix: reduce, scan tile sums, apply ------------
 // Element i of the sequence is mass[perm[i]] (or mass[i] when perm is null).
 __device__ __forceinline__ double seq_elem(const double *mass, const u32 *perm,
-                                           u32 i) {
-  return perm ? mass[perm[i]] : mass[i];
+                                           u32 i, u32 n) {
+  return perm ? mass[sorted_slot(perm, i, n)] : mass[i];
 }
 
 __device__ double block_excl_scan_f64(double v, double *s_w, double *total) {
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(
   const u32 base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   double acc = 0.0;
   for (int j = 0; j < SCAN_ITEMS; ++j)
-    if (base + j < n) acc = acc + seq_elem(mass, perm, base + j);
+    if (base + j < n) acc = acc + seq_elem(mass, perm, base + j, n);
   double tot;
   (void)block_excl_scan_f64(acc, s_w, &tot);
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    vals[j] = (base + j < n) ? seq_elem(mass, perm, base + j) : 0.0;
+    vals[j] = (base + j < n) ? seq_elem(mass, perm, base + j, n) : 0.0;
     acc = acc + vals[j];
   }
   double tot;
@@ -278,7 +280,8 @@ __global__ __launch_bounds__(BLOCK) void k_energy_terms(
     const double v = vel[i];
     const double mr = mass ? mass[i] / mscale : mq / mscale;
     const double C = (cum ? cum[i] : (double)i * mq) / mscale;
-    const double M = mtot / mscale;
+    // unequal masses: the total is the last prefix plus the last mass
+    const double M = (cum ? cum[n - 1] + mass[n - 1] : mtot) / mscale;
     e = twopiG * mr * x * ((2.0 * C + mr) - M);
     e = e + mr * (v * v) / 2.0;
     if (omega2 >= 0.0) e = e + mr * omega2 * (x * x) / 2.0;
@@ -298,13 +301,16 @@ __global__ __launch_bounds__(BLOCK) void k_momentum_terms(
 }
 
 // Deterministic final sum of block partials (fixed order).
+// out[0] = sum of the partials; NaN when *err (a sort the sum depends on
+// gave up) is set.
 __global__ __launch_bounds__(BLOCK) void k_sum_partials(
-    u32 np, const double *__restrict__ partial, double *__restrict__ out) {
+    u32 np, const double *__restrict__ partial, double *__restrict__ out,
+    const u32 *__restrict__ err) {
   __shared__ double s_w[BLOCK / 64];
   double acc = 0.0;
   for (u32 j = threadIdx.x; j < np; j += BLOCK) acc = acc + partial[j];
   const double t = block_sum_f64(acc, s_w);
-  if (threadIdx.x == 0) out[0] = t;
+  if (threadIdx.x == 0) out[0] = (err && *err) ? __builtin_nan("") : t;
 }
 
 // out[k] = [omega2 * y_k^2 / 2] + twopiG * sum_j m_j |x_j - y_k|
@@ -355,7 +361,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather2(
     double *__restrict__ bo) {
   const u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  const u32 s = perm[i];
+  const u32 s = sorted_slot(perm, i, n);
   ao[i] = a[s];
   if (b) bo[i] = b[s];
 }
@@ -480,59 +486,107 @@ int launch_desort(wdy_sim *s, double *x_out, double *v_out, const double *acc,
   return WDY_OK;
 }
 
+// Scratch layout of energy_enqueue for n sheets (unequal masses when
+// `masses`): sorted keys (two buffers), permutation (two), gathered
+// velocities, gathered masses and their prefix, tile sums and block
+// partials, then the radix sort's histograms and tickets.  (The sort's
+// look-back words are passed separately: they must stay where earlier
+// calls left them.)
+struct EnergyLayout {
+  size_t k0, k1, p0, p1, sv, sm, cum, ts, hist, counters, bytes;
+  u32 nt, nb, ntiles;
+};
+
+static EnergyLayout energy_layout(u32 n, bool masses) {
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  EnergyLayout L{};
+  L.nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  L.nb = grid_for(n);
+  L.ntiles = radix_tiles(n);
+  size_t o = 0;
+  L.k0 = o; o += up((size_t)n * 8);
+  L.k1 = o; o += up((size_t)n * 8);
+  L.p0 = o; o += up((size_t)n * 4);
+  L.p1 = o; o += up((size_t)n * 4);
+  L.sv = o; o += up((size_t)n * 8);
+  L.sm = o; if (masses) o += up((size_t)n * 8);
+  L.cum = o; if (masses) o += up((size_t)n * 8);
+  L.ts = o; o += up(((size_t)L.nt + L.nb + 1) * 8);
+  L.hist = o; o += up(radix_hist_bytes());
+  L.counters = o; o += up(radix_counter_bytes());
+  L.bytes = o;
+  return L;
+}
+
+size_t energy_scratch_bytes(u32 n, bool masses) {
+  return energy_layout(n, masses).bytes;
+}
+
+int energy_enqueue(hipStream_t st, u32 n, const double *x, const double *v,
+                   const double *m, double mq, double mscale, double twopiG,
+                   double omega2, double *e_dev, u64 *status, void *ws,
+                   u32 *epoch, const u32 **err_word) {
+  const EnergyLayout L = energy_layout(n, m != nullptr);
+  char *w = (char *)ws;
+  u64 *k0 = (u64 *)(w + L.k0), *k1 = (u64 *)(w + L.k1);
+  u32 *p0 = (u32 *)(w + L.p0), *p1 = (u32 *)(w + L.p1);
+  double *sv = (double *)(w + L.sv);
+  double *sm = m ? (double *)(w + L.sm) : nullptr;
+  double *cum = m ? (double *)(w + L.cum) : nullptr;
+  double *ts = (double *)(w + L.ts), *red = ts;
+  RadixWork rw;
+  rw.n = n;
+  rw.ntiles = L.ntiles;
+  rw.hist = (u32 *)(w + L.hist);
+  rw.counters = (u32 *)(w + L.counters);
+  rw.status = status;
+  rw.epoch = *epoch;
+  rw.clear_err = true;
+  if (err_word) *err_word = radix_err_word(&rw);
+  k_encode<<<L.nb, BLOCK, 0, st>>>(n, x, k0);
+  WDY_CHECK_LAUNCH();
+  const int rc = radix_sort_pairs(&rw, st, nullptr, k0, k1, p0, p1, true);
+  *epoch = rw.epoch;
+  if (rc) return rc;
+  k_gather2<<<L.nb, BLOCK, 0, st>>>(n, p0, v, m, sv, sm);
+  if (m) {
+    // exclusive prefix of the sorted masses; total = C[n-1] + m[n-1]
+    k_scan_reduce<<<L.nt, BLOCK, 0, st>>>(n, sm, nullptr, ts);
+    k_scan_top<<<1, BLOCK, 0, st>>>(L.nt, ts);
+    k_scan_apply<<<L.nt, BLOCK, 0, st>>>(n, sm, nullptr, ts, cum);
+  }
+  k_energy_terms<<<L.nb, BLOCK, 0, st>>>(n, k0, sv, sm, cum, mq,
+                                         (double)n * mq, mscale, twopiG, omega2,
+                                         red);
+  k_sum_partials<<<1, BLOCK, 0, st>>>(L.nb, red, e_dev, radix_err_word(&rw));
+  WDY_CHECK_LAUNCH();
+  return WDY_OK;
+}
+
 int energy_core(hipStream_t st, u32 n, const double *x, const double *v,
                 const double *m, double mq, double mscale, double twopiG,
                 double omega2, double *e_out) {
   DevBufs b;
-  double *sv, *sm = nullptr, *cum = nullptr, *ts, *red;
-  u64 *k0, *k1;
-  u32 *p0, *p1;
-  const u32 nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-  const u32 nb = grid_for(n);
-  WDY_HIP(b.alloc(&sv, n));
-  if (m) {
-    WDY_HIP(b.alloc(&sm, n));
-    WDY_HIP(b.alloc(&cum, n));
-  }
-  WDY_HIP(b.alloc(&ts, nt + nb + 1));
-  red = ts;
-  WDY_HIP(b.alloc(&k0, 2 * (size_t)n));
-  k1 = k0 + n;
-  WDY_HIP(b.alloc(&p0, 2 * (size_t)n));
-  p1 = p0 + n;
-  RadixWork rw;
-  int rc = radix_alloc(&rw, n);
-  if (!rc) {
-    k_encode<<<nb, BLOCK, 0, st>>>(n, x, k0);
-    rc = radix_sort_pairs(&rw, st, nullptr, k0, k1, p0, p1, true);
-  }
-  if (!rc) {
-    k_gather2<<<nb, BLOCK, 0, st>>>(n, p0, v, m, sv, sm);
-    double mtot = (double)n * mq;
-    if (m) {
-      // exclusive prefix of the sorted masses; total = C[n-1] + m[n-1]
-      k_scan_reduce<<<nt, BLOCK, 0, st>>>(n, sm, nullptr, ts);
-      k_scan_top<<<1, BLOCK, 0, st>>>(nt, ts);
-      k_scan_apply<<<nt, BLOCK, 0, st>>>(n, sm, nullptr, ts, cum);
-      double cl = 0.0, ml = 0.0;
-      if (hipMemcpyAsync(&cl, cum + n - 1, 8, hipMemcpyDeviceToHost, st) ||
-          hipMemcpyAsync(&ml, sm + n - 1, 8, hipMemcpyDeviceToHost, st) ||
-          hipStreamSynchronize(st))
-        rc = fail(WDY_ERUNTIME, "energy: mass total copy failed");
-      mtot = cl + ml;
-    }
-    if (!rc) {
-      k_energy_terms<<<nb, BLOCK, 0, st>>>(n, k0, sv, sm, cum, mq, mtot, mscale,
-                                           twopiG, omega2, red);
-      k_sum_partials<<<1, BLOCK, 0, st>>>(nb, red, red + nb);
-      if (hipGetLastError() != hipSuccess ||
-          hipMemcpyAsync(e_out, red + nb, 8, hipMemcpyDeviceToHost, st) ||
-          hipStreamSynchronize(st))
-        rc = fail(WDY_ERUNTIME, "energy: kernel or copy failed");
-    }
-  }
-  radix_free(&rw);
-  return rc;
+  char *ws;
+  double *e_dev;
+  u64 *status;
+  WDY_HIP(b.alloc(&ws, energy_scratch_bytes(n, m != nullptr)));
+  WDY_HIP(b.alloc(&e_dev, 1));
+  WDY_HIP(b.alloc(&status, radix_status_bytes(n) / 8));
+  // fresh look-back words: zero, then tagged by epochs from 1
+  WDY_HIP(hipMemsetAsync(status, 0, radix_status_bytes(n), st));
+  u32 epoch = 1;
+  const u32 *err_word = nullptr;
+  if (int rc = energy_enqueue(st, n, x, v, m, mq, mscale, twopiG, omega2, e_dev,
+                              status, ws, &epoch, &err_word))
+    return rc;
+  u32 gave_up = 0;
+  WDY_HIP(hipMemcpyAsync(e_out, e_dev, 8, hipMemcpyDeviceToHost, st));
+  WDY_HIP(hipMemcpyAsync(&gave_up, err_word, 4, hipMemcpyDeviceToHost, st));
+  WDY_HIP(hipStreamSynchronize(st));
+  if (gave_up)
+    return fail(WDY_ERUNTIME, "energy: a look-back wait of the sort gave up");
+  return WDY_OK;
 }
 
 int launch_energy(wdy_sim *s, double twopiG, double omega2, double *e_out) {
@@ -575,7 +629,7 @@ int launch_momentum_arrays(int64_t n64, const double *v, const double *m,
   WDY_HIP(hipMemcpy(dv, v, 8 * (size_t)n, hipMemcpyHostToDevice));
   WDY_HIP(hipMemcpy(dm, m, 8 * (size_t)n, hipMemcpyHostToDevice));
   k_momentum_terms<<<nb, BLOCK>>>(n, dv, dm, red);
-  k_sum_partials<<<1, BLOCK>>>(nb, red, red + nb);
+  k_sum_partials<<<1, BLOCK>>>(nb, red, red + nb, nullptr);
   WDY_CHECK_LAUNCH();
   WDY_HIP(hipMemcpy(p_out, red + nb, 8, hipMemcpyDeviceToHost));
   return WDY_OK;

@@ -13,7 +13,7 @@ constexpr int WARM_WMAX = 64;          // largest window (buckets either side)
 // across the stale splitters; over 2400 steps of the N = 2^24 disk, 1536-
 // sheet buckets never overfilled at targets up to 1260 and did at 1280.
 struct WarmShape {
-  int threads, cap, nq;
+  int threads, cap, nq;  // nq: 32-bit words of the sort's bins
   // widest window the partition's per-wave segment counters (nq u32 shared
   // with the sort's bins) can hold: waves * (2W + 2) <= nq
   int wmax() const {

@@ -1018,13 +1018,14 @@ int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
 
 }  // namespace
 
-// Tile shapes (threads per workgroup, bucket capacity in sheets, position
-// bins).  Shapes 4-6 keep the LDS footprint small enough for two or three
-// workgroups per CU, so one workgroup's memory phases overlap another's sort.
+// Tile shapes (threads per workgroup, bucket capacity in sheets, 32-bit
+// words of position bins).  Shapes 4-6 keep the LDS footprint small enough
+// for two or three workgroups per CU, so one workgroup's memory phases
+// overlap another's sort; shape 8 packs two 16-bit bins per word (2048 bins).
 static const WarmShape kShapes[WARM_NSHAPES] = {
     {1024, 4096, 4096}, {512, 2048, 2048}, {256, 1024, 1024}, {512, 1024, 1024},
     {512, 2048, 1024}, {512, 3072, 1024}, {768, 3072, 768},
-    {256, 2048, 1024}, {256, 1536, 768}, {256, 1792, 1024}};
+    {256, 2048, 1024}, {256, 1536, 1024}, {256, 1792, 1024}};
 
 const WarmShape &warm_shape(int v) { return kShapes[v]; }
 
@@ -1039,7 +1040,7 @@ int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
     case 5: return launch_shape<Shape<512, 3072, 1024>>(st, a, equal_mass, harm, mode);
     case 6: return launch_shape<Shape<768, 3072, 768>>(st, a, equal_mass, harm, mode);
     case 7: return launch_shape<Shape<256, 2048, 1024>>(st, a, equal_mass, harm, mode);
-    case 8: return launch_shape<Shape<256, 1536, 768, 4>>(st, a, equal_mass, harm, mode);
+    case 8: return launch_shape<Shape<256, 1536, 2048, 4, true>>(st, a, equal_mass, harm, mode);
     default: return launch_shape<Shape<256, 1792, 1024>>(st, a, equal_mass, harm, mode);
   }
 }
