@@ -235,6 +235,25 @@ int wdy_energy_individual_arrays(int64_t n, const double *x, const double *v,
                                  const double *m, double twopiG, double omega2,
                                  double *out);
 
+/* Allocation-free form of wdy_energy_arrays, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  x, v, m (raw masses, as for
+ * wdy_energy_arrays) and e_out are DEVICE pointers; the computation is
+ * enqueued on `stream` (a hipStream_t; NULL: the default stream) and the
+ * call returns without waiting: *e_out holds the energy, bit for bit the
+ * value wdy_energy_arrays returns for the same arrays, once the stream has
+ * reached it (NaN if a bounded wait of its sort gave up, which does not
+ * happen in normal operation).  Every scratch buffer comes from one static
+ * device array of
+ * about 51 MiB that each GPU the library runs on holds until the process
+ * ends; nothing is allocated or freed.  n must be 1 .. WDY_NOALLOC_MAX_N
+ * (WDY_EINVAL beyond).  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams, other threads) race: serialise them. */
+#define WDY_NOALLOC_MAX_N (1 << 20)
+int wdy_energy_device_noalloc(int64_t n, const double *x, const double *v,
+                              const double *m, double twopiG, double omega2,
+                              double *e_out, void *stream);
+
 /* wendy.momentum(v, m) (wendy/wendy.py:477-491). */
 int wdy_momentum_arrays(int64_t n, const double *v, const double *m,
                         double *p_out);

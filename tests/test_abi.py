@@ -18,6 +18,7 @@ HEADER = os.path.join(ROOT, 'include', 'wendy_amd.h')
 def header_functions():
     text = open(HEADER).read()
     text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    text = re.sub(r'^\s*#.*$', '', text, flags=re.M)  # preprocessor lines
     names = re.findall(r'\b(\w+)\s*\([^;{]*\)\s*;', text)
     return sorted(set(n for n in names
                       if n.startswith('wdy_') or n.startswith('_wendy')

@@ -37,6 +37,7 @@ EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
+            'wdy_energy_device_noalloc',
             'wdy_set_ext_force', 'wdy_set_ext_force_host', 'wdy_time', 'wdy_warm_failures', 'wdy_warm_config',
             'wdy_read_and_advance', 'wdy_host_alloc', 'wdy_host_free',
             'wdy_copy_probe',
@@ -103,6 +104,10 @@ def _declare(lib):
     lib.wdy_energy_individual_arrays.argtypes = [
         ctypes.c_int64, _F64, _F64, _F64, ctypes.c_double, ctypes.c_double,
         _F64W]
+    lib.wdy_energy_device_noalloc.restype = ctypes.c_int
+    lib.wdy_energy_device_noalloc.argtypes = [ctypes.c_int64, S, S, S,
+                                              ctypes.c_double, ctypes.c_double,
+                                              S, S]
     lib.wdy_momentum_arrays.restype = ctypes.c_int
     lib.wdy_momentum_arrays.argtypes = [ctypes.c_int64, _F64, _F64,
                                         _c_double_p]
