@@ -117,6 +117,24 @@ __device__ __forceinline__ u32 seg_pos(const u32 *segpre, const u32 *segsrc,
   return segsrc[lo] + (e - segpre[lo]);
 }
 
+// Smallest power of two >= x (x >= 1).
+__host__ __device__ constexpr u32 pow2_ge(u32 x) {
+  return x <= 1 ? 1u : 1u << (32 - __builtin_clz(x - 1));
+}
+
+// The largest a < P with tab[a] <= v, for a table that is non-decreasing on
+// [0, P) with tab[0] <= v (P a power of two <= 256; entries past the real
+// ones padded with the type's maximum): a fixed sequence of halving steps,
+// one load, compare and select each, the same for every lane.
+template <class T>
+__device__ __forceinline__ u32 pow2_search(const T *tab, u32 P, T v) {
+  u32 a = 0;
+#pragma unroll 1
+  for (u32 st = P >> 1; st; st >>= 1)
+    if (tab[a + st] <= v) a += st;
+  return a;
+}
+
 template <class SH, bool EQ, bool HARM, int MODE>
 __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
   constexpr int WT = SH::WT, PER = SH::PER, CAP = SH::CAP, NWAVE = SH::NWAVE;
@@ -481,16 +499,18 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
 }
 
 // LDS bytes of the pipelined kernel's tables for S segments, placed after
-// the bucket arrays: two splitter windows (S+1 u64 each: the current
-// bucket's, read by its partition, and the next one's), then one segment
-// table (S sources, S+1 prefixes), the partition's S+1 output prefixes and
-// LOVF overflow indices.  Only the splitter window is needed by two buckets
-// at once: the next bucket's segment table and overflow indices are read
-// when its loads are issued, before the current bucket's partition, and
-// again (over capacity) at its own commit, before the following table
-// overwrites them.
+// the bucket arrays: two splitter windows (the current bucket's, read by its
+// partition, and the next one's), then one segment table (S sources and
+// the segments' start offsets), the partition's S+1 output prefixes and
+// LOVF overflow indices.  The splitter windows (S+1 entries) and the start
+// offsets (S) are padded to pow2_ge(S + 1) entries with the type's maximum,
+// so both are searched with fixed halving steps.  Only the splitter window
+// is needed by two buckets at once: the next bucket's segment table and
+// overflow indices are read when its loads are issued, before the current
+// bucket's partition, and again (over capacity) at its own commit, before
+// the following table overwrites them.
 __host__ __device__ constexpr size_t pipe_table_bytes(u32 S) {
-  return 16ull * (S + 1) + 4ull * (S + 2 * (S + 1) + LOVF);
+  return 16ull * pow2_ge(S + 1) + 4ull * (S + pow2_ge(S + 1) + (S + 1) + LOVF);
 }
 
 // Persistent, software-pipelined form of mode 0.  One workgroup per CU loops
@@ -511,12 +531,14 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
   extern __shared__ u64 lds_dyn[];
   Lds<SH, EQ> s(lds_dyn);
   const u32 S = A.S, W = A.W, nb = A.nb;
-  u64 *const s_spl0 = lds_dyn + Lds<SH, EQ>::BYTES / 8;  // [2][S + 1]
-  u32 *const s_segsrc = (u32 *)(s_spl0 + 2 * (S + 1));   // [S]
-  u32 *const s_segpre = s_segsrc + S;                    // [S + 1]
-  u32 *const s_segout = s_segpre + S + 1;                // [S + 1]
+  const u32 TP = pow2_ge(S + 1);  // padded table length
+  u64 *const s_spl0 = lds_dyn + Lds<SH, EQ>::BYTES / 8;  // [2][TP]
+  u32 *const s_segsrc = (u32 *)(s_spl0 + 2 * TP);        // [S]
+  u32 *const s_segpre = s_segsrc + S;                    // [TP]
+  u32 *const s_segout = s_segpre + TP;                   // [S + 1]
   u32 *const s_oidx = s_segout + S + 1;                  // [LOVF]
-  auto s_spl = [&](int bb) { return s_spl0 + (u32)bb * (S + 1); };
+  auto s_spl = [&](int bb) { return s_spl0 + (u32)bb * TP; };
+  const u32 PSEG = pow2_ge(S);  // the segment search's span
   __shared__ u32 s_novf;
   __shared__ u32 s_tot;
   __shared__ u64 s_red[NWAVE];
@@ -526,6 +548,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
+  // padding of the start offsets (the scans write only [0, S))
+  for (u32 q = S + t; q < TP; q += WT) s_segpre[q] = 0xffffffffu;
   if (t == 0) {
     s_u[0] = atomicAdd(A.tick, 1u);
     s_u[1] = ld_u32(A.err);
@@ -568,6 +592,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       s_segpre[t] = tb_cnt;
     }
     if (t <= hi - lo + 1) s_spl(bb)[t] = tb_spl;
+    else if (t < TP) s_spl(bb)[t] = ~0ull;  // padding past the window
     for (u32 q2 = t; q2 < novf_all; q2 += WT)
       if (A.rd.odst[q2] == q) {
         const u32 slot = atomicAdd(&s_novf, 1u);
@@ -591,7 +616,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     for (int i = 0; i < PER; ++i) {
       const u32 e = i * WT + t;
       if (e < nseg) {
-        const u32 pos = seg_pos(s_segpre, s_segsrc, S, e);
+        const u32 d = pow2_search(s_segpre, PSEG, e);
+        const u32 pos = s_segsrc[d] + (e - s_segpre[d]);
         dk[i] = A.kin[pos];
         dv[i] = A.vin[pos];
         di[i] = A.iin[pos];
@@ -860,13 +886,9 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           const bool below = lo > 0 && kp < spl[0];
           const bool above = hi + 1 < nb && kp >= spl[hi - lo + 1];
           if (!below && !above) {
-            // the largest a with spl[a] <= kp (bisection: the same trip
-            // count for every lane of the wave)
-            u32 a = 0, z = hi - lo;
-            while (a < z) {
-              const u32 mid = (a + z + 1) >> 1;
-              if (spl[mid] <= kp) a = mid; else z = mid - 1;
-            }
+            // the largest a with spl[a] <= kp: spl[0] <= kp < spl[hi-lo+1],
+            // and the window is padded past hi-lo+1
+            const u32 a = pow2_search(spl, pow2_ge(hi - lo + 1), kp);
             d = lo + a + W - j;
           } else {
             const u32 dst = bucket_of(A.rd.spl, nb, kp);
