@@ -127,6 +127,31 @@ int wdy_energy(wdy_sim *sim, double twopiG, double omega2, double *e_out);
 /* Free the simulation's device buffers and stream. */
 int wdy_destroy(wdy_sim *sim);
 
+/* Gangs: several simulations advanced together, one kernel launch per
+ * leapfrog step for all of them, so that many medium simulations fill the
+ * GPU the way one large one does (serving, ensembles of large members).
+ * No reference counterpart: the reference advances one simulation per
+ * generator (wendy/wendy.py:424-437). */
+typedef struct wdy_gang wdy_gang;
+
+/* A gang on the current HIP device, with its own stream. */
+int wdy_gang_create(wdy_gang **out);
+
+/* Waits for the gang's work and frees it.  Its members stay valid. */
+int wdy_gang_destroy(wdy_gang *gang);
+
+/* Enqueue one output interval of each of sims[0..count) (1 <= count <= 64,
+ * distinct, on the gang's device): nleap[i] leapfrog steps of h[i], with the
+ * results and the error handling of wdy_advance(sims[i], h[i], nleap[i]),
+ * bit for bit.  Members whose next interval runs on the warm-start kernel
+ * (large path, already partitioned for h[i], one of the automatically
+ * chosen tile shapes, no host-function external force) run in the gang's
+ * launches; the others run their interval on their own streams.
+ * Asynchronous like wdy_advance: each member's interval is checked by the
+ * next call on that member. */
+int wdy_gang_advance(wdy_gang *gang, wdy_sim *const *sims, int count,
+                     const double *h, const int *nleap);
+
 /* Warm-start tuning of a large-path sim.
  *   shape          tile shape of the warm kernel (-1 keeps the current one):
  *                  0: 1024 threads, buckets of up to 4096 sheets
@@ -135,7 +160,7 @@ int wdy_destroy(wdy_sim *sim);
  *                  with 1024 position bins;  5: 512 threads, up to 3072
  *                  with 1024 bins;  6: 768 threads, up to 3072 with 768
  *                  bins;  7: 256 threads, up to 2048 with 1024 bins;
- *                  8: 256 threads, up to 1536 with 768 bins (the
+ *                  8: 256 threads, up to 1536 with 2048 bins (the
  *                  default);  9: 256 threads, up to 1792 with 1024 bins
  *   bucket_target  sheets per value bucket, WDY_BUCKET_MIN .. the shape's
  *                  capacity; 0 leaves 1/5 of the capacity (at least 256

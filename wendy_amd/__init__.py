@@ -7,6 +7,6 @@ include/wendy_amd.h); there is no CPU fallback.
 """
 from . import forces  # noqa: F401
 from .batch import Batch  # noqa: F401
-from .nbody import Sim, energy, momentum, nbody, potential  # noqa: F401
+from .nbody import Gang, Sim, energy, momentum, nbody, potential  # noqa: F401
 
 __version__ = '0.1.0'

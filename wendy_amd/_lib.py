@@ -34,6 +34,7 @@ WDY_NO_WARM = 8
 EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',
             'wdy_create', 'wdy_advance', 'wdy_read', 'wdy_energy',
             'wdy_destroy', 'wdy_tune', 'wdy_path_stats',
+            'wdy_gang_create', 'wdy_gang_destroy', 'wdy_gang_advance',
             'wdy_set_timing', 'wdy_kernel_time',
             'wdy_energy_arrays', 'wdy_energy_individual_arrays',
             'wdy_momentum_arrays', 'wdy_potential_arrays',
@@ -88,6 +89,13 @@ def _declare(lib):
     lib.wdy_energy.argtypes = [S, ctypes.c_double, ctypes.c_double, _c_double_p]
     lib.wdy_destroy.restype = ctypes.c_int
     lib.wdy_destroy.argtypes = [S]
+    lib.wdy_gang_create.restype = ctypes.c_int
+    lib.wdy_gang_create.argtypes = [ctypes.POINTER(S)]
+    lib.wdy_gang_destroy.restype = ctypes.c_int
+    lib.wdy_gang_destroy.argtypes = [S]
+    lib.wdy_gang_advance.restype = ctypes.c_int
+    lib.wdy_gang_advance.argtypes = [S, ctypes.POINTER(S), ctypes.c_int,
+                                     _F64, ctypes.POINTER(ctypes.c_int)]
     lib.wdy_tune.restype = ctypes.c_int
     lib.wdy_tune.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.wdy_path_stats.restype = ctypes.c_int

@@ -249,7 +249,7 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     WDY_HIP(hipMalloc(&s->red, (N / 256 + 4) * sizeof(double)));
     WDY_HIP(hipMalloc(&s->out_x, N * sizeof(double)));
     WDY_HIP(hipMalloc(&s->out_v, N * sizeof(double)));
-    int r = radix_alloc(&s->radix, (u32)n);
+    int r = radix_alloc(&s->radix, (u32)n, s->stream);
     if (r) return r;
     // staging: x -> out_x, v -> out_v, m -> cum, sheet ids -> idx[1]
     WDY_HIP(hipMemcpyAsync(s->out_x, x_slot, N * 8, hipMemcpyHostToDevice,
@@ -648,8 +648,71 @@ int wdy_destroy(wdy_sim *sim) {
   if (!sim) return WDY_OK;
   (void)hipSetDevice(sim->device);
   (void)hipStreamSynchronize(sim->stream);
+  warm_wait(sim);  // an interval may run on a gang's stream
+  (void)hipEventSynchronize(sim->ev1);
   free_sim(sim);
   return WDY_OK;
+}
+
+int wdy_gang_create(wdy_gang **out) {
+  if (!out) return fail(WDY_EINVAL, "wdy_gang_create: out is NULL");
+  *out = nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return fail(WDY_ENODEV, "wdy_gang_create: no GPU visible");
+  WarmGang *g = nullptr;
+  if (int rc = warm_gang_create(dev, &g)) return rc;
+  *out = (wdy_gang *)g;
+  return WDY_OK;
+}
+
+int wdy_gang_destroy(wdy_gang *gang) {
+  warm_gang_destroy((WarmGang *)gang);
+  return WDY_OK;
+}
+
+int wdy_gang_advance(wdy_gang *gang, wdy_sim *const *sims, int count,
+                     const double *h, const int *nleap) {
+  if (!gang || !sims || !h || !nleap)
+    return fail(WDY_EINVAL, "wdy_gang_advance: NULL argument");
+  if (count < 1 || count > WARM_GANG_MAX)
+    return fail(WDY_EINVAL, "wdy_gang_advance: count=%d outside [1, %d]",
+                count, WARM_GANG_MAX);
+  WarmGang *g = (WarmGang *)gang;
+  const int dev = warm_gang_device(g);
+  for (int i = 0; i < count; ++i) {
+    if (!sims[i]) return fail(WDY_EINVAL, "wdy_gang_advance: sims[%d] is NULL", i);
+    if (nleap[i] < 1)
+      return fail(WDY_EINVAL, "wdy_gang_advance: nleap[%d]=%d < 1", i, nleap[i]);
+    if (!std::isfinite(h[i]))
+      return fail(WDY_EINVAL, "wdy_gang_advance: h[%d] not finite", i);
+    if (sims[i]->device != dev)
+      return fail(WDY_EINVAL, "wdy_gang_advance: sims[%d] is on device %d, "
+                  "the gang on %d", i, sims[i]->device, dev);
+    for (int q = 0; q < i; ++q)
+      if (sims[q] == sims[i])
+        return fail(WDY_EINVAL, "wdy_gang_advance: sims[%d] repeats sims[%d]",
+                    i, q);
+  }
+  // settle every member (a failed interval is redone now), then run the
+  // members whose next interval is not warm on their own streams
+  std::vector<wdy_sim *> gs;
+  std::vector<double> gh;
+  std::vector<int> gn;
+  for (int i = 0; i < count; ++i) {
+    wdy_sim *s = sims[i];
+    if (int rc = settle(s)) return rc;
+    if (warm_gang_eligible(s, h[i])) {
+      gs.push_back(s);
+      gh.push_back(h[i]);
+      gn.push_back(nleap[i]);
+    } else if (int rc = advance_impl(s, h[i], nleap[i], nullptr, nullptr)) {
+      return rc;
+    }
+  }
+  if (gs.empty()) return WDY_OK;
+  WDY_HIP(hipSetDevice(dev));
+  return warm_gang_interval(g, gs.data(), (int)gs.size(), gh.data(), gn.data());
 }
 
 int wdy_tune(wdy_sim *sim, int shape, int bucket_target, int window) {

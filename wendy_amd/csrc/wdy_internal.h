@@ -149,7 +149,8 @@ u32 radix_tiles(u32 n);
 size_t radix_hist_bytes();
 size_t radix_counter_bytes();
 size_t radix_status_bytes(u32 n);  // look-back words: zero before first use
-int radix_alloc(RadixWork *w, u32 n);
+// zeroes its counters and look-back words on stream st
+int radix_alloc(RadixWork *w, u32 n, hipStream_t st);
 // Source slot of sorted position i.  A sort whose look-back wait gave up can
 // leave slots of its payload unwritten (arbitrary values); clamping keeps
 // every access that follows inside the arrays until the host reports the
@@ -299,6 +300,19 @@ int warm_settle(wdy_sim *s, bool *failed, double *h, int *nleap);
 int warm_current_shape(wdy_sim *s);
 // the warm path's tile shape, bucket target and window; false without one
 bool warm_config(wdy_sim *s, int *shape, u32 *T, u32 *W);
+// wait (without checking) for the warm interval in flight, if any
+void warm_wait(wdy_sim *s);
+// gangs: several simulations' warm intervals, one launch per leapfrog step
+// for all of them, on the gang's own stream (wdy_warm.hip)
+struct WarmGang;
+int warm_gang_create(int device, WarmGang **out);
+void warm_gang_destroy(WarmGang *g);
+int warm_gang_device(const WarmGang *g);
+// the sim's next interval of step h can run in a gang: it is warm-ready for
+// h, settled, on a gang-compiled shape and has no host-function force
+bool warm_gang_eligible(wdy_sim *s, double h);
+int warm_gang_interval(WarmGang *g, wdy_sim *const *sims, int count,
+                       const double *h, const int *nleap);
 int launch_energy_arrays(int64_t n, const double *x, const double *v,
                          const double *m, double twopiG, double omega2,
                          double *e_out);

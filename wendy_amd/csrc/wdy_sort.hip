@@ -245,15 +245,16 @@ size_t radix_status_bytes(u32 n) {
   return (size_t)radix_tiles(n) * RADIX * sizeof(u64);
 }
 
-int radix_alloc(RadixWork *w, u32 n) {
+int radix_alloc(RadixWork *w, u32 n, hipStream_t st) {
   w->n = n;
   w->ntiles = (n + TILE - 1) / TILE;
   if (w->ntiles == 0) w->ntiles = 1;
   WDY_HIP(hipMalloc(&w->hist, PASSES * RADIX * sizeof(u32)));
   WDY_HIP(hipMalloc(&w->counters, 64 * sizeof(u32)));
-  WDY_HIP(hipMemset(w->counters, 0, 64 * sizeof(u32)));
+  WDY_HIP(hipMemsetAsync(w->counters, 0, 64 * sizeof(u32), st));
   WDY_HIP(hipMalloc(&w->status, (size_t)w->ntiles * RADIX * sizeof(u64)));
-  WDY_HIP(hipMemset(w->status, 0, (size_t)w->ntiles * RADIX * sizeof(u64)));
+  WDY_HIP(hipMemsetAsync(w->status, 0, (size_t)w->ntiles * RADIX * sizeof(u64),
+                         st));
   w->epoch = 1;
   return WDY_OK;
 }

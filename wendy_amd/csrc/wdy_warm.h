@@ -93,5 +93,11 @@ int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
 // zero the ticket ring, the overflow-count ring except slot `keep` (4: none),
 // and the error word
 int warm_reset_launch(hipStream_t st, u32 *ring, u32 keep, u32 *err);
+// shapes the gang kernel is compiled for (the default, the wide one, shape 0:
+// every shape the automatic window fitting and back-off choose)
+bool warm_gang_shape(int shape);
+// most simulations one gang launch can hold (one ballot over their ticket
+// starts)
+constexpr int WARM_GANG_MAX = 64;
 
 }  // namespace wdy

@@ -42,7 +42,9 @@
 // exceeds its bounded wait) records the failing launch's epoch in err[0];
 // later launches exit at once, and the host redoes the interval from its
 // untouched start state on the radix path.
+#include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -135,8 +137,14 @@ __device__ __forceinline__ u32 pow2_search(const T *tab, u32 P, T v) {
   return a;
 }
 
+// One bucket of mode 0 or 1 (steps 2-7 above) for ticket j of the step
+// described by A; novf_in: the overflow count of the list it reads.  s_u
+// (4 words of the caller's LDS) has s_u[3] = 0 and is published by a
+// barrier on entry.  Every thread of the workgroup calls it with the same
+// arguments.
 template <class SH, bool EQ, bool HARM, int MODE>
-__global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
+__device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
+                                            const u32 novf_in, u32 *s_u) {
   constexpr int WT = SH::WT, PER = SH::PER, CAP = SH::CAP, NWAVE = SH::NWAVE;
   extern __shared__ u64 lds_dyn[];
   Lds<SH, EQ> s(lds_dyn);
@@ -146,30 +154,11 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
   __shared__ u32 s_oidx[LOVF];
   __shared__ u64 s_red[NWAVE];
   __shared__ u64 s_mm[2 * NWAVE];
-  __shared__ u32 s_u[4];
   __shared__ u64 s_u64[2];
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   const u32 S = A.S, W = A.W, nb = A.nb;
   const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
-  // ---- 1. ticket, error word and overflow count: one round trip ------------
-  if (t == 0) {
-    s_u[0] = atomicAdd(A.tick, 1u);
-    s_u[1] = ld_u32(A.err);
-    s_u[3] = 0;  // overflow entries destined to this bucket
-  }
-  const u32 novf_in = MODE == 0 ? ld_u32(A.ovf_r) : 0u;
-  __syncthreads();
-  const u32 j = s_u[0];
-  if (s_u[1] != 0 && s_u[1] < A.epoch) return;  // an earlier launch failed
-  if (j >= nb) {  // tickets start at 0 each launch; never expected
-    if (t == 0) set_err(A.err, A.epoch, WARM_ERR_STALL);
-    return;
-  }
-  if (j == 0 && t == 0) {  // clear the slots launch epoch+2 will use
-    *A.tick_next2 = 0;
-    *A.ovf_next2 = 0;
-  }
   WARM_STAMP(0);
   const u32 lo = j >= W ? j - W : 0u;
   const u32 hi = j + W < nb ? j + W : nb - 1;
@@ -496,6 +485,63 @@ __global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
     }
   }
   WARM_STAMP(5);
+}
+
+template <class SH, bool EQ, bool HARM, int MODE>
+__global__ __launch_bounds__(SH::WT) void k_warm_step(WarmArgs A) {
+  __shared__ u32 s_u[4];
+  const u32 t = threadIdx.x;
+  // ---- 1. ticket, error word and overflow count: one round trip ------------
+  if (t == 0) {
+    s_u[0] = atomicAdd(A.tick, 1u);
+    s_u[1] = ld_u32(A.err);
+    s_u[3] = 0;  // overflow entries destined to this bucket
+  }
+  const u32 novf_in = MODE == 0 ? ld_u32(A.ovf_r) : 0u;
+  __syncthreads();
+  const u32 j = s_u[0];
+  if (s_u[1] != 0 && s_u[1] < A.epoch) return;  // an earlier launch failed
+  if (j >= A.nb) {  // tickets start at 0 each launch; never expected
+    if (t == 0) set_err(A.err, A.epoch, WARM_ERR_STALL);
+    return;
+  }
+  if (j == 0 && t == 0) {  // clear the slots launch epoch+2 will use
+    *A.tick_next2 = 0;
+    *A.ovf_next2 = 0;
+  }
+  warm_bucket<SH, EQ, HARM, MODE>(A, j, novf_in, s_u);
+}
+
+// Gang form of mode 0: one launch runs one leapfrog step of several
+// simulations, args[i] describing simulation i's step (its own epoch,
+// look-back words, error word and overflow rings).  Gang tickets
+// bstart[i] .. bstart[i+1]-1 are simulation i's buckets in order, so a
+// bucket's look-back only ever waits on lower tickets, as in k_warm_step.
+// nsim <= 64: each wave finds its simulation with one ballot over bstart.
+template <class SH, bool EQ, bool HARM>
+__global__ __launch_bounds__(SH::WT) void k_warm_gang(
+    const WarmArgs *__restrict__ args, const u32 *__restrict__ bstart,
+    u32 nsim, u32 *tick, u32 *tick_next2) {
+  __shared__ u32 s_u[4];
+  const u32 t = threadIdx.x, lane = t & 63;
+  if (t == 0) s_u[0] = atomicAdd(tick, 1u);
+  __syncthreads();
+  const u32 g = s_u[0];
+  if (g >= bstart[nsim]) return;  // the grid is the ticket count; never expected
+  if (g == 0 && t == 0) *tick_next2 = 0;
+  const u64 le = __ballot(lane < nsim && bstart[lane] <= g);
+  const u32 i = __builtin_amdgcn_readfirstlane((u32)__popcll(le) - 1u);
+  const WarmArgs &A = args[i];
+  const u32 j = g - bstart[i];
+  if (t == 0) {
+    s_u[1] = ld_u32(A.err);
+    s_u[3] = 0;
+  }
+  const u32 novf_in = ld_u32(A.ovf_r);
+  __syncthreads();
+  if (s_u[1] != 0 && s_u[1] < A.epoch) return;  // an earlier step failed
+  if (j == 0 && t == 0) *A.ovf_next2 = 0;
+  warm_bucket<SH, EQ, HARM, 0>(A, j, novf_in, s_u);
 }
 
 // LDS bytes of the pipelined kernel's tables for S segments, placed after
@@ -1038,6 +1084,53 @@ int launch_shape(hipStream_t st, const WarmArgs &a, bool eq, bool harm,
               : launch_one<SH, false, false, 1>(st, a);
 }
 
+template <class SH, bool EQ, bool HARM>
+int launch_gang(hipStream_t st, const WarmArgs *args, const u32 *bstart,
+                u32 nsim, u32 total, u32 *tick, u32 *tick_next2) {
+  auto fn = k_warm_gang<SH, EQ, HARM>;
+  const size_t lds = Lds<SH, EQ>::BYTES;
+  if (lds > 65536)
+    WDY_HIP(hipFuncSetAttribute((const void *)fn,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  fn<<<total, SH::WT, lds, st>>>(args, bstart, nsim, tick, tick_next2);
+  WDY_CHECK_LAUNCH();
+  return WDY_OK;
+}
+
+template <class SH>
+int launch_gang_shape(hipStream_t st, bool eq, bool harm, const WarmArgs *args,
+                      const u32 *bstart, u32 nsim, u32 total, u32 *tick,
+                      u32 *tick_next2) {
+  if (eq)
+    return harm ? launch_gang<SH, true, true>(st, args, bstart, nsim, total, tick, tick_next2)
+                : launch_gang<SH, true, false>(st, args, bstart, nsim, total, tick, tick_next2);
+  return harm ? launch_gang<SH, false, true>(st, args, bstart, nsim, total, tick, tick_next2)
+              : launch_gang<SH, false, false>(st, args, bstart, nsim, total, tick, tick_next2);
+}
+
+// Per-member reset of a gang interval (k_warm_reset's work for each member)
+// and the gang's own ticket ring.
+struct GangReset {
+  u32 *ring;
+  u32 *err;
+  u32 keep;
+  u32 pad;
+};
+
+__global__ void k_gang_reset(const GangReset *r, u32 count, u32 *gring) {
+  const u32 t = threadIdx.x;
+  if (t < 4) gring[t] = 0;
+  for (u32 i = t; i < count; i += blockDim.x) {
+    u32 *ring = r[i].ring;
+    for (u32 q = 0; q < 4; ++q) ring[q] = 0;
+    for (u32 q = 0; q < 4; ++q)
+      if (q != r[i].keep) ring[4 + q] = 0;
+    r[i].err[0] = 0;
+    r[i].err[1] = 0;
+  }
+}
+
 }  // namespace
 
 // Tile shapes (threads per workgroup, bucket capacity in sheets, 32-bit
@@ -1071,6 +1164,28 @@ int warm_reset_launch(hipStream_t st, u32 *ring, u32 keep, u32 *err) {
   k_warm_reset<<<1, 64, 0, st>>>(ring, keep, err);
   WDY_CHECK_LAUNCH();
   return WDY_OK;
+}
+
+bool warm_gang_shape(int shape) {
+  return shape == WARM_SHAPE_DEFAULT || shape == WARM_SHAPE_WIDE || shape == 0;
+}
+
+static int warm_gang_launch(hipStream_t st, int shape, bool eq, bool harm,
+                            const WarmArgs *args, const u32 *bstart, u32 nsim,
+                            u32 total, u32 *tick, u32 *tick_next2) {
+  switch (shape) {
+    case WARM_SHAPE_DEFAULT:
+      return launch_gang_shape<Shape<256, 1536, 2048, 4, true>>(
+          st, eq, harm, args, bstart, nsim, total, tick, tick_next2);
+    case WARM_SHAPE_WIDE:
+      return launch_gang_shape<Shape<256, 2048, 1024>>(
+          st, eq, harm, args, bstart, nsim, total, tick, tick_next2);
+    case 0:
+      return launch_gang_shape<Shape<1024, 4096>>(st, eq, harm, args, bstart,
+                                                  nsim, total, tick, tick_next2);
+    default:
+      return fail(WDY_EUNSUPPORTED, "gang launch: shape %d not compiled", shape);
+  }
 }
 
 }  // namespace wdy
@@ -1171,9 +1286,9 @@ int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
     s->warm = new (std::nothrow) WarmCtx();
     if (!s->warm) return fail(WDY_ENOMEM, "warm context allocation failed");
     WDY_HIP(hipMalloc(&s->warm->err, 8));
-    WDY_HIP(hipMemset(s->warm->err, 0, 8));
+    WDY_HIP(hipMemsetAsync(s->warm->err, 0, 8, s->stream));
     WDY_HIP(hipMalloc(&s->warm->ring, 32));
-    WDY_HIP(hipMemset(s->warm->ring, 0, 32));
+    WDY_HIP(hipMemsetAsync(s->warm->ring, 0, 32, s->stream));
     WDY_HIP(hipHostMalloc((void **)&s->warm->h_err, 8, hipHostMallocDefault));
     WDY_HIP(hipEventCreateWithFlags(&s->warm->ev_err, hipEventDisableTiming));
     // fixed-point scale: 2^k * totmass <= 2^62
@@ -1203,7 +1318,7 @@ int warm_configure(wdy_sim *s, int shape, u32 T, u32 W) {
     for (auto &m : w.meta)
       if (int rc = meta_alloc(m, nseg, nbp1)) return rc;
     WDY_HIP(hipMalloc(&w.status, nbp1 * 5 * 8));
-    WDY_HIP(hipMemset(w.status, 0, nbp1 * 5 * 8));
+    WDY_HIP(hipMemsetAsync(w.status, 0, nbp1 * 5 * 8, s->stream));
     w.seg_alloc = nseg;
     w.nb_alloc = nbp1;
   }
@@ -1376,61 +1491,91 @@ bool warm_ready_for(wdy_sim *s, double h) {
   return s->warm && s->warm->ready && s->warm->hp == h / 2.;
 }
 
+// The buffer sets of one warm interval: it starts from state set `start`
+// and metadata set `mstart` and writes only the two other sets of each
+// (three rotate), so a failure found later can still rewind to its start.
+struct IntervalPlan {
+  int start = 0, mstart = 0, so[2] = {0, 0}, mo[2] = {0, 0};
+  int in = 0, rdm = 0;  // the sets the next step reads
+  double t = 0.0;       // force time of the next step (wendy/wendy.c:401-408)
+};
+
+static IntervalPlan plan_begin(wdy_sim *s) {
+  IntervalPlan p;
+  p.start = s->cur;
+  p.mstart = s->warm->rd;
+  int ns = 0, nm = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (i != p.start) p.so[ns++] = i;
+    if (i != p.mstart) p.mo[nm++] = i;
+  }
+  p.in = p.start;
+  p.rdm = p.mstart;
+  p.t = s->t0;
+  return p;
+}
+
+// Arguments of step k (0 <= k < nleap) of the interval; advances the plan.
+static WarmArgs plan_step(wdy_sim *s, IntervalPlan &p, int k, int nleap,
+                          double h) {
+  WarmCtx &w = *s->warm;
+  const int out = p.so[k & 1], wrm = p.mo[k & 1];
+  WarmArgs a = base_args(s);
+  set_state(a, s, p.in, out);
+  a.rd = w.meta[p.rdm];
+  a.wr = w.meta[wrm];
+  set_epoch(a, w, ++w.epoch);
+  a.h = h;
+  a.pre = k == 0 ? h / 2. : 0.0;
+  a.hd = k == nleap - 1 ? h / 2. : h;
+  a.hp = k == nleap - 1 ? h / 2. : 0.0;
+  a.t = p.t;
+  p.t = p.t + h;
+  a.prof = w.prof;
+  p.in = out;
+  p.rdm = wrm;
+  return a;
+}
+
+// After the interval's launches (and the copy of its error word to h_err,
+// marked by ev_err, on the stream they ran on): the interval is pending and
+// the sim's state is the one it leaves when it succeeds.
+static void plan_end(wdy_sim *s, const IntervalPlan &p, double h, int nleap) {
+  WarmCtx &w = *s->warm;
+  w.pending = true;
+  w.p_start = p.start;
+  w.p_mstart = p.mstart;
+  w.p_h = h;
+  w.p_nleap = nleap;
+  w.p_t0 = s->t0;
+  s->cur = p.in;
+  w.rd = p.rdm;
+  if (s->ext.kind) s->t0 = p.t;
+}
+
 // Enqueues one output interval of nleap warm steps and returns without
 // waiting for it: the device's error word is copied to the host behind the
-// interval and checked by warm_settle.  The state and metadata sets the
-// interval starts from are not written by it (three sets rotate), so a
-// failure found later can still rewind to them.
+// interval and checked by warm_settle.
 int warm_interval(wdy_sim *s, double h, int nleap) {
   WarmCtx &w = *s->warm;
   if (w.pending) return fail(WDY_ERUNTIME, "warm interval enqueued before "
                              "the previous one was checked");
-  const int start = s->cur, mstart = w.rd;
-  int so[2], mo[2], ns = 0, nm = 0;
-  for (int i = 0; i < 3; ++i) {
-    if (i != start) so[ns++] = i;
-    if (i != mstart) mo[nm++] = i;
-  }
+  IntervalPlan p = plan_begin(s);
   // keep the overflow count of the list the first launch reads (written by
   // the launch with the current epoch)
   if (int rc = warm_reset_launch(s->stream, w.ring, w.epoch & 3u, w.err))
     return rc;
-  int in = start, rdm = mstart;
-  double t = s->t0;  // force time of step k (wendy/wendy.c:401-408)
   for (int k = 0; k < nleap; ++k) {
-    const int out = so[k & 1], wrm = mo[k & 1];
-    WarmArgs a = base_args(s);
-    set_state(a, s, in, out);
-    a.rd = w.meta[rdm];
-    a.wr = w.meta[wrm];
-    set_epoch(a, w, ++w.epoch);
-    a.h = h;
-    a.pre = k == 0 ? h / 2. : 0.0;
-    a.hd = k == nleap - 1 ? h / 2. : h;
-    a.hp = k == nleap - 1 ? h / 2. : 0.0;
-    a.t = t;
-    t = t + h;
-    a.prof = w.prof;
+    const WarmArgs a = plan_step(s, p, k, nleap, h);
     if (s->timing) timer_begin(s->timer, s->stream, KC_WARM);
     int rc = warm_step_launch(s->stream, a, s->equal_mass, s->omega2 >= 0.0,
                               w.pipe ? 2 : 0, w.shape);
     if (s->timing) timer_end(s->timer, s->stream, KC_WARM);
     if (rc) return rc;
-    in = out;
-    rdm = wrm;
   }
   WDY_HIP(hipMemcpyAsync(w.h_err, w.err, 8, hipMemcpyDeviceToHost, s->stream));
   WDY_HIP(hipEventRecord(w.ev_err, s->stream));
-  w.pending = true;
-  w.p_start = start;
-  w.p_mstart = mstart;
-  w.p_h = h;
-  w.p_nleap = nleap;
-  w.p_t0 = s->t0;
-  // the state as it will be when the interval succeeds
-  s->cur = in;
-  w.rd = rdm;
-  if (s->ext.kind) s->t0 = t;
+  plan_end(s, p, h, nleap);
   return WDY_OK;
 }
 
@@ -1495,6 +1640,182 @@ int warm_settle(wdy_sim *s, bool *failed, double *h, int *nleap) {
     return WDY_OK;
   }
   return warm_configure(s, shape, T, W);
+}
+
+
+// ---- gangs: one launch per leapfrog step for several simulations ------------
+struct WarmGang {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  u32 *ring = nullptr;  // [4] ticket counters of the gang's launches
+  u32 epoch = 0;
+  unsigned char *h_buf = nullptr;  // pinned staging of one interval's tables
+  unsigned char *d_buf = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev_up = nullptr;  // the last upload from h_buf has completed
+};
+
+int warm_gang_create(int device, WarmGang **out) {
+  WarmGang *g = new (std::nothrow) WarmGang();
+  if (!g) return fail(WDY_ENOMEM, "gang allocation failed");
+  g->device = device;
+  int rc = [&]() -> int {
+    WDY_HIP(hipSetDevice(device));
+    // not synchronised with the null stream: allocations and uploads of
+    // other simulations do not stall the gang's steps
+    WDY_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    WDY_HIP(hipMalloc(&g->ring, 16));
+    WDY_HIP(hipMemsetAsync(g->ring, 0, 16, g->stream));
+    WDY_HIP(hipEventCreateWithFlags(&g->ev_up, hipEventDisableTiming));
+    WDY_HIP(hipStreamSynchronize(g->stream));
+    return WDY_OK;
+  }();
+  if (rc) {
+    warm_gang_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return WDY_OK;
+}
+
+void warm_gang_destroy(WarmGang *g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->ring) (void)hipFree(g->ring);
+  if (g->d_buf) (void)hipFree(g->d_buf);
+  if (g->h_buf) (void)hipHostFree(g->h_buf);
+  if (g->ev_up) (void)hipEventDestroy(g->ev_up);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+int warm_gang_device(const WarmGang *g) { return g->device; }
+
+bool warm_gang_eligible(wdy_sim *s, double h) {
+  return !s->small && !s->host_force && warm_ready_for(s, h) &&
+         !s->warm->pending && warm_gang_shape(s->warm->shape);
+}
+
+void warm_wait(wdy_sim *s) {
+  if (s->warm && s->warm->pending) (void)hipEventSynchronize(s->warm->ev_err);
+}
+
+// Enqueues one output interval of every member (nleap[i] steps of h[i]) on
+// the gang's stream: step k is one launch per (shape, mass kind, harmonic)
+// group of the members that still have a step k, each bucket of each
+// member one gang ticket.  Each member's arguments are those its own
+// warm_interval would use (same epochs, buffer sets and clock), so its
+// results are the same bit for bit; each member's error word is copied to
+// its host word behind the interval and checked by its own warm_settle.
+// Every member must satisfy warm_gang_eligible(h[i]) and be on the gang's
+// device; the members' own streams must be idle.
+int warm_gang_interval(WarmGang *g, wdy_sim *const *sims, int count,
+                       const double *h, const int *nleap) {
+  struct Member {
+    wdy_sim *s;
+    IntervalPlan p;
+    int key;  // group: shape, mass kind, harmonic term
+  };
+  struct Launch {
+    int key;
+    size_t args_off, bst_off;
+    u32 nsim, total;
+  };
+  std::vector<Member> m((size_t)count);
+  int kmax = 0;
+  for (int i = 0; i < count; ++i) {
+    wdy_sim *s = sims[i];
+    m[i].s = s;
+    m[i].key = s->warm->shape * 4 + (s->equal_mass ? 2 : 0) + (s->omega2 >= 0.0 ? 1 : 0);
+    kmax = nleap[i] > kmax ? nleap[i] : kmax;
+  }
+  std::vector<int> keys;
+  for (const Member &q : m)
+    if (std::find(keys.begin(), keys.end(), q.key) == keys.end()) keys.push_back(q.key);
+  // resets first: they keep the overflow slot of each member's current epoch
+  std::vector<GangReset> resets((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    WarmCtx &w = *m[i].s->warm;
+    resets[i].ring = w.ring;
+    resets[i].err = w.err;
+    resets[i].keep = w.epoch & 3u;
+    resets[i].pad = 0;
+    m[i].p = plan_begin(m[i].s);
+  }
+  std::vector<WarmArgs> args;
+  std::vector<u32> bst;
+  std::vector<Launch> launches;
+  for (int k = 0; k < kmax; ++k) {
+    for (int key : keys) {
+      Launch L;
+      L.key = key;
+      L.args_off = args.size();
+      L.bst_off = bst.size();
+      L.nsim = 0;
+      bst.push_back(0u);
+      for (int i = 0; i < count; ++i) {
+        if (m[i].key != key || nleap[i] <= k) continue;
+        WarmArgs a = plan_step(m[i].s, m[i].p, k, nleap[i], h[i]);
+        a.prof = nullptr;
+        args.push_back(a);
+        bst.push_back(bst.back() + a.nb);
+        L.nsim += 1;
+      }
+      if (L.nsim == 0) {
+        bst.pop_back();
+        continue;
+      }
+      L.total = bst.back();
+      launches.push_back(L);
+    }
+  }
+  // one upload: resets, then every launch's arguments and ticket starts
+  const size_t r_bytes = (resets.size() * sizeof(GangReset) + 255) & ~(size_t)255;
+  const size_t a_bytes = (args.size() * sizeof(WarmArgs) + 255) & ~(size_t)255;
+  const size_t bytes = r_bytes + a_bytes + bst.size() * sizeof(u32);
+  WDY_HIP(hipSetDevice(g->device));
+  WDY_HIP(hipEventSynchronize(g->ev_up));  // h_buf free again
+  if (bytes > g->cap) {
+    if (g->d_buf) (void)hipFree(g->d_buf);
+    if (g->h_buf) (void)hipHostFree(g->h_buf);
+    g->d_buf = g->h_buf = nullptr;
+    g->cap = 0;
+    const size_t cap = bytes * 2;
+    WDY_HIP(hipMalloc(&g->d_buf, cap));
+    WDY_HIP(hipHostMalloc((void **)&g->h_buf, cap, hipHostMallocDefault));
+    g->cap = cap;
+  }
+  memcpy(g->h_buf, resets.data(), resets.size() * sizeof(GangReset));
+  memcpy(g->h_buf + r_bytes, args.data(), args.size() * sizeof(WarmArgs));
+  memcpy(g->h_buf + r_bytes + a_bytes, bst.data(), bst.size() * sizeof(u32));
+  WDY_HIP(hipMemcpyAsync(g->d_buf, g->h_buf, bytes, hipMemcpyHostToDevice,
+                         g->stream));
+  WDY_HIP(hipEventRecord(g->ev_up, g->stream));
+  const WarmArgs *d_args = (const WarmArgs *)(g->d_buf + r_bytes);
+  const u32 *d_bst = (const u32 *)(g->d_buf + r_bytes + a_bytes);
+  for (const Member &q : m) WDY_HIP(hipEventRecord(q.s->ev0, g->stream));
+  k_gang_reset<<<1, 256, 0, g->stream>>>((const GangReset *)g->d_buf,
+                                         (u32)count, g->ring);
+  WDY_CHECK_LAUNCH();
+  for (const Launch &L : launches) {
+    const u32 e = ++g->epoch;
+    const int shape = L.key / 4;
+    if (int rc = warm_gang_launch(g->stream, shape, (L.key & 2) != 0,
+                                  (L.key & 1) != 0, d_args + L.args_off,
+                                  d_bst + L.bst_off, L.nsim, L.total,
+                                  g->ring + (e & 3u), g->ring + ((e + 2) & 3u)))
+      return rc;
+  }
+  for (int i = 0; i < count; ++i) {
+    wdy_sim *s = m[i].s;
+    WarmCtx &w = *s->warm;
+    WDY_HIP(hipMemcpyAsync(w.h_err, w.err, 8, hipMemcpyDeviceToHost, g->stream));
+    WDY_HIP(hipEventRecord(w.ev_err, g->stream));
+    WDY_HIP(hipEventRecord(s->ev1, g->stream));
+    plan_end(s, m[i].p, h[i], nleap[i]);
+  }
+  return WDY_OK;
 }
 
 }  // namespace wdy

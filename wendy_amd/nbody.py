@@ -21,6 +21,50 @@ from .forces import ExtForce
 _sort_type_dict = {'quick': 0, 'merge': 1, 'tim': 2, 'qsort': 3, 'parallel': 4}
 
 
+class Gang:
+    """Several large-path simulations advanced together (a ``wdy_gang``):
+    one kernel launch per leapfrog step for all members whose next interval
+    is a warm-start one, on the gang's own stream.  ``advance(sims, h,
+    nleap)`` gives each member the result of ``sim.advance(h[i], nleap[i])``
+    bit for bit; ``h`` and ``nleap`` may be scalars.  At most 64 members per
+    call."""
+
+    MAX = 64
+
+    def __init__(self):
+        self._g = None
+        lib = _lib.lib()
+        self._destroy = lib.wdy_gang_destroy
+        g = ctypes.c_void_p()
+        _lib.check(lib.wdy_gang_create(ctypes.byref(g)))
+        self._g = g
+
+    def advance(self, sims, h, nleap):
+        k = len(sims)
+        if k == 0:
+            return
+        hs = numpy.ascontiguousarray(numpy.broadcast_to(
+            numpy.asarray(h, dtype=numpy.float64), (k,)))
+        ns = numpy.broadcast_to(numpy.asarray(nleap, dtype=numpy.int64), (k,))
+        handles = (ctypes.c_void_p * k)(*[s._h.value for s in sims])
+        cn = (ctypes.c_int * k)(*[int(q) for q in ns])
+        _lib.check(_lib.lib().wdy_gang_advance(self._g, handles, k, hs, cn))
+
+    def close(self, _finalizing=sys.is_finalizing):
+        if self._g is not None and not _finalizing():
+            self._destroy(self._g)
+        self._g = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Sim:
     """One device-resident approx simulation (a ``wdy_sim`` handle).
 
