@@ -220,17 +220,25 @@ def serve_workload(window_s, rate=200., seed=0, pool_log2=21):
             for i in range(k)]
 
 
+# largest request the serving pool holds (one workgroup per simulation);
+# larger ones advance in gangs on the whole-GPU path.  Measured over a 1 s
+# window of the serving load: split 2^13 served 2.2e10 sheet-steps/s, 2^12
+# 2.0e10, 2^16 (the pool then bounded by its largest, one-workgroup members)
+# 0.97e10.
+SERVE_SPLIT = 1 << 13
+
+
 def bench_serve(args, log):
     """Continuous batching of the serving load on one GPU (wendy_amd.serve):
-    a 200 GB HBM budget, pool of 2048 slots for N <= 2^16, larger requests on
-    the large path (up to 8 at once), ticks of 10 steps."""
+    a 200 GB HBM budget, pool of 2048 slots for N <= 2^13, larger requests on
+    the large path in two gangs (up to 128 at once), intervals of 10 steps."""
     from wendy_amd import serve
     reqs = serve_workload(args.serve_window, args.serve_rate)
     offered = sum(r.n * r.steps for r in reqs) / args.serve_window
     log('serve: %d requests over %.1f s (offered %.3e sheet-steps/s)'
         % (len(reqs), args.serve_window, offered))
-    s = serve.Server(budget_bytes=200e9, max_sims=2048, split=1 << 16,
-                     max_big=8, tick=10)
+    s = serve.Server(budget_bytes=200e9, max_sims=2048, split=SERVE_SPLIT,
+                     tick=10, large='gang', ngangs=2)
     try:
         res = s.run(reqs)
     finally:
@@ -242,7 +250,8 @@ def bench_serve(args, log):
     return {'workload': ('Poisson arrivals %.0f/s for %.1f s, N log-uniform '
                          '[2^10, 2^20], lognormal(0,0.3) masses, z_h~U(0.5,2), '
                          'sigma=sqrt(z_h), omega~U(0,0.2), 100-10000 steps '
-                         'log-uniform, h=0.01, 200 GB HBM budget'
+                         'log-uniform, h=0.01, 200 GB HBM budget; pool for '
+                         'N <= 2^13, larger N in two gangs'
                          % (args.serve_rate, args.serve_window)),
             'requests': len(reqs), 'offered_sheet_steps_per_s': offered,
             'p50_latency_s': summ['p50_s'], 'p99_latency_s': summ['p99_s'],

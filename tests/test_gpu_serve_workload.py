@@ -3,11 +3,12 @@
 lognormal(0, 0.3) masses; omega ~ U(0, 0.2); h = 0.01), drawn from the same
 generator the benchmark uses (bench.serve_workload), over a short arrival
 window and with the step counts capped so the run stays short; both kinds of
-request are present: pool-resident (n <= 2^16) and large-path (up to 2^20,
-several at once, their warm kernels running beside the pool's).
+request are present: pool-resident (n <= 2^13) and large-path (up to 2^20,
+advancing together in gangs beside the pool's launches).
 
 Each result is compared bit for bit with the same request served alone (a
-fresh server holding only it), each request of n <= 2^14 with the oracle
+fresh server holding only it, a large-path request in a Sim of its own),
+each request of n <= 2^14 with the oracle
 (the same sequence of output intervals) at ||.||_inf relative 1e-10, and
 every result's energy with its initial energy."""
 import numpy
@@ -30,10 +31,14 @@ def capped(reqs):
                           omega=r.omega, arrival=r.arrival) for r in reqs]
 
 
+SPLIT = bench.SERVE_SPLIT
+
+
 def server():
-    # the benchmark's configuration (bench.bench_serve)
-    return serve.Server(budget_bytes=200e9, max_sims=2048, split=1 << 16,
-                        max_big=8, tick=TICK)
+    # the benchmark's configuration (bench.bench_serve): large-path requests
+    # advance together in gangs
+    return serve.Server(budget_bytes=200e9, max_sims=2048, split=SPLIT,
+                        tick=TICK, large='gang', ngangs=2)
 
 
 def oracle_result(r):
@@ -64,16 +69,18 @@ def test_serving_load_alone_equals_mix_and_oracle(gpu_lib):
     print('mix served', s.stats, flush=True)
     for r in reqs:
         got = mix[r.id]
-        assert got.where == ('pool' if r.n <= 1 << 16 else 'sim')
+        assert got.where == ('pool' if r.n <= SPLIT else 'sim')
         if got.where == 'sim':
             warm, cold, redo, cap, ovf, stall = got.paths
             # the first interval sorts from scratch, later ones run warm
             # (an interval the device rejects is redone on the radix path)
             assert warm + cold == r.steps, got.paths
             assert stall == 0 and (r.steps <= TICK or warm > 0), got.paths
-        # alone: a fresh server (a small pool suffices) holding only it
+        # alone: a fresh server (a small pool suffices) holding only it, a
+        # large-path request in a Sim of its own
         alone_s = serve.Server(budget_bytes=200e9, pool_sheets=1 << 17,
-                               max_sims=4, split=1 << 16, max_big=1, tick=TICK)
+                               max_sims=4, split=SPLIT, max_big=1, tick=TICK,
+                               large='threads')
         try:
             alone = alone_s.run([serve.Request(r.id, r.n, r.make, r.steps, r.h,
                                                omega=r.omega)])[r.id]

@@ -64,26 +64,49 @@ class FakeBig:
         return self.x + sum(self.ks), numpy.full(n, float(len(self.ks)))
 
 
+class FakeGang:
+    """Advances its members one interval each per call, as wdy_gang_advance
+    does (at most MAX members per call)."""
+    MAX = 3
+    calls = []
+
+    def advance(self, bigs, hs, ks):
+        assert 1 <= len(bigs) <= self.MAX and len(set(map(id, bigs))) == len(bigs)
+        FakeGang.calls.append(len(bigs))
+        for b, h, k in zip(bigs, hs, ks):
+            b.advance(h, k)
+
+    def close(self):
+        pass
+
+
 def req(i, n, steps, arrival=0.):
     return serve.Request(i, n, lambda n=n, i=i: (numpy.full(n, float(i)),
                                                  numpy.zeros(n), numpy.ones(n)),
                          steps, 0.01, arrival=arrival)
 
 
-def make_server(cap=10_000, slots=4, **kw):
+@pytest.fixture(params=['gang', 'threads'])
+def large(request):
+    return request.param
+
+
+def make_server(cap=10_000, slots=4, large='gang', **kw):
     pools = []
 
     def pf():
         pools.append(FakePool(cap, slots))
         return pools[0]
     FakeBig.made = []
+    FakeGang.calls = []
     s = serve.Server(budget_bytes=kw.pop('budget', 1e12), pool_sheets=cap,
-                     max_sims=slots, pool_factory=pf, big_factory=FakeBig, **kw)
+                     max_sims=slots, pool_factory=pf, big_factory=FakeBig,
+                     gang_factory=FakeGang, large=large, **kw)
     return s, pools[0]
 
 
-def test_every_request_gets_its_steps_in_ticks():
-    s, pool = make_server(split=100, tick=10)
+def test_every_request_gets_its_steps_in_ticks(large):
+    s, pool = make_server(split=100, tick=10, large=large)
     reqs = [req(0, 50, 25), req(1, 100, 10), req(2, 500, 7), req(3, 101, 31),
             req(4, 1, 1)]
     res = s.run(reqs)
@@ -118,11 +141,11 @@ def test_arena_capacity_limits_admission():
     assert len(res) == 3 and pool.max_live == 1
 
 
-def test_budget_limits_large_simulations():
+def test_budget_limits_large_simulations(large):
     per = serve.SIM_BYTES_PER_SHEET
     pool_bytes = 100 * serve.POOL_BYTES_PER_SHEET
     s, _ = make_server(cap=100, split=10, budget=pool_bytes + 2.5 * 1000 * per,
-                       max_big=8, tick=4)
+                       max_big=8, tick=4, large=large)
     res = s.run([req(i, 1000, 8) for i in range(5)])
     assert len(res) == 5 and s.stats['admission_stalls'] > 0
     # at most two 1000-sheet simulations fit beside the arena at once
@@ -132,8 +155,9 @@ def test_budget_limits_large_simulations():
         assert live <= 2
 
 
-def test_request_larger_than_budget_is_refused():
-    s, _ = make_server(cap=100, split=10, budget=100 * serve.POOL_BYTES_PER_SHEET + 10)
+def test_request_larger_than_budget_is_refused(large):
+    s, _ = make_server(cap=100, split=10, budget=100 * serve.POOL_BYTES_PER_SHEET + 10,
+                       large=large)
     with pytest.raises(MemoryError):
         s.run([req(0, 1000, 3)])
 
@@ -150,3 +174,17 @@ def test_latency_summary():
     res = s.run([req(i, 10, 2 + i) for i in range(10)])
     summ = serve.latency_summary(res)
     assert summ['completed'] == 10 and summ['p99_s'] >= summ['p50_s'] >= 0.
+
+
+def test_gang_members_capped_per_call():
+    """More large-path simulations than one gang call takes: they advance in
+    arrival order, at most MAX per call, each by exactly its steps."""
+    s, _ = make_server(split=10, tick=3, max_big=10, large='gang')
+    reqs = [req(i, 100 + i, 7 + 2 * i) for i in range(8)]
+    res = s.run(reqs)
+    assert sorted(res) == list(range(8))
+    for r in reqs:
+        assert numpy.all(res[r.id].x == r.id + r.steps)
+        assert res[r.id].where == 'sim'
+    assert FakeGang.calls and max(FakeGang.calls) <= FakeGang.MAX
+    assert s.stats['gang_intervals'] == len(FakeGang.calls)

@@ -449,6 +449,7 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
                         double *cum_last) {
   WDY_HIP(hipSetDevice(s->device));
   if (int rc = settle(s)) return rc;
+  warm_gang_release(s);  // ev0/ev1 bracket this interval
   WDY_HIP(hipEventRecord(s->ev0, s->stream));
   if (s->host_force) {
     if (int rc = advance_host_force(s, h, nleap, acc_last, cum_last)) return rc;
@@ -469,6 +470,16 @@ static int advance_impl(wdy_sim *s, double h, int nleap, double *acc_last,
   }
   WDY_HIP(hipEventRecord(s->ev1, s->stream));
   return WDY_OK;
+}
+
+// Device seconds of the last interval: its own events, or its gang
+// interval's when it ran in a gang.
+static float interval_ms(wdy_sim *s) {
+  hipEvent_t a = s->ev0, b = s->ev1;
+  (void)warm_gang_events(s, &a, &b);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
+  return ms;
 }
 
 static int check_sim(wdy_sim *s, const char *fn) {
@@ -530,11 +541,7 @@ int wdy_read(wdy_sim *sim, double *x_out, double *v_out, double *seconds_out) {
                              N * 8, hipMemcpyDeviceToHost, sim->stream));
   }
   WDY_HIP(hipStreamSynchronize(sim->stream));
-  if (seconds_out) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, sim->ev0, sim->ev1) != hipSuccess) ms = 0.f;
-    *seconds_out = ms * 1e-3;
-  }
+  if (seconds_out) *seconds_out = interval_ms(sim) * 1e-3;
   return WDY_OK;
 }
 
@@ -550,11 +557,7 @@ int wdy_read_and_advance(wdy_sim *sim, double *x_out, double *v_out, double h,
   WDY_HIP(hipSetDevice(sim->device));
   const size_t N = (size_t)sim->n;
   WDY_HIP(hipEventSynchronize(sim->ev1));  // the interval being read
-  if (seconds_out) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, sim->ev0, sim->ev1) != hipSuccess) ms = 0.f;
-    *seconds_out = ms * 1e-3;
-  }
+  if (seconds_out) *seconds_out = interval_ms(sim) * 1e-3;
   if (x_out || v_out) {
     if (int rc = launch_desort(sim, sim->out_x, sim->out_v, nullptr, nullptr, nullptr))
       return rc;

@@ -302,6 +302,10 @@ int warm_current_shape(wdy_sim *s);
 bool warm_config(wdy_sim *s, int *shape, u32 *T, u32 *W);
 // wait (without checking) for the warm interval in flight, if any
 void warm_wait(wdy_sim *s);
+// the events bracketing the last interval when it ran in a gang
+bool warm_gang_events(wdy_sim *s, hipEvent_t *start, hipEvent_t *done);
+// the next interval runs outside a gang: drop the settled gang record
+void warm_gang_release(wdy_sim *s);
 // gangs: several simulations' warm intervals, one launch per leapfrog step
 // for all of them, on the gang's own stream (wdy_warm.hip)
 struct WarmGang;

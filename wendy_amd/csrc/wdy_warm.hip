@@ -45,6 +45,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <memory>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -1118,6 +1119,14 @@ struct GangReset {
   u32 pad;
 };
 
+// out[2 i], out[2 i + 1] := member i's error words (behind its interval)
+__global__ void k_gang_errs(const GangReset *r, u32 count, u32 *out) {
+  for (u32 i = threadIdx.x; i < count; i += blockDim.x) {
+    out[2 * i] = r[i].err[0];
+    out[2 * i + 1] = r[i].err[1];
+  }
+}
+
 __global__ void k_gang_reset(const GangReset *r, u32 count, u32 *gring) {
   const u32 t = threadIdx.x;
   if (t < 4) gring[t] = 0;
@@ -1195,6 +1204,26 @@ static int warm_gang_launch(hipStream_t st, int shape, bool eq, bool harm,
 // ============================================================================
 namespace wdy {
 
+// Completion of one gang interval, shared by its members until each has
+// checked its error words: `start` is recorded before the interval's first
+// launch; the members' error words are gathered on the device, copied to
+// h_err[2 i], h_err[2 i + 1] for member i, and `done` recorded behind the
+// copy.  One event and one copy per interval, whatever the member count.
+struct GangRecord {
+  int device = 0;
+  hipEvent_t start = nullptr, done = nullptr;
+  u32 *h_err = nullptr;  // pinned [2 * WARM_GANG_MAX]
+  u32 *d_err = nullptr;  // [2 * WARM_GANG_MAX]
+  ~GangRecord() {
+    (void)hipSetDevice(device);
+    if (done) (void)hipEventSynchronize(done);
+    if (start) (void)hipEventDestroy(start);
+    if (done) (void)hipEventDestroy(done);
+    if (h_err) (void)hipHostFree(h_err);
+    if (d_err) (void)hipFree(d_err);
+  }
+};
+
 struct WarmCtx {
   u32 T = 0, W = 0, S = 0, nb = 0, cap = 0;
   bool pipe = true;  // mode 0 by the persistent pipelined kernel
@@ -1211,6 +1240,10 @@ struct WarmCtx {
   // copied to h_err and ev_err marks the copy's completion; until checked,
   // the interval's start set and metadata (p_start, p_mstart) stay intact
   hipEvent_t ev_err = nullptr;
+  // when the last interval ran in a gang: the gang interval's record (its
+  // events and this member's error words at rec_idx) instead of ev_err/h_err
+  std::shared_ptr<GangRecord> rec;
+  int rec_idx = 0;
   bool pending = false;
   int p_start = 0, p_mstart = 0, p_nleap = 0;
   double p_h = 0.0, p_t0 = 0.0;
@@ -1561,6 +1594,7 @@ int warm_interval(wdy_sim *s, double h, int nleap) {
   if (w.pending) return fail(WDY_ERUNTIME, "warm interval enqueued before "
                              "the previous one was checked");
   IntervalPlan p = plan_begin(s);
+  w.rec.reset();  // checked through ev_err / h_err
   // keep the overflow count of the list the first launch reads (written by
   // the launch with the current epoch)
   if (int rc = warm_reset_launch(s->stream, w.ring, w.epoch & 3u, w.err))
@@ -1589,14 +1623,22 @@ int warm_settle(wdy_sim *s, bool *failed, double *h, int *nleap) {
   WarmCtx *wp = s->warm;
   if (!wp || !wp->pending) return WDY_OK;
   WarmCtx &w = *wp;
-  WDY_HIP(hipEventSynchronize(w.ev_err));
+  u32 e0, kind;
+  if (w.rec) {
+    WDY_HIP(hipEventSynchronize(w.rec->done));
+    e0 = w.rec->h_err[2 * w.rec_idx];
+    kind = w.rec->h_err[2 * w.rec_idx + 1];
+  } else {
+    WDY_HIP(hipEventSynchronize(w.ev_err));
+    e0 = w.h_err[0];
+    kind = w.h_err[1];
+  }
   w.pending = false;
-  if (w.h_err[0] == 0) {
-    if (w.prof) prof_fold(s);
+  if (e0 == 0) {
+    if (w.prof && !w.rec) prof_fold(s);
     s->warm_steps += w.p_nleap;
     return WDY_OK;
   }
-  const u32 kind = w.h_err[1];
   *failed = true;
   *h = w.p_h;
   *nleap = w.p_nleap;
@@ -1653,7 +1695,26 @@ struct WarmGang {
   unsigned char *d_buf = nullptr;
   size_t cap = 0;
   hipEvent_t ev_up = nullptr;  // the last upload from h_buf has completed
+  // interval records; one is free again once no member refers to it
+  std::vector<std::shared_ptr<GangRecord>> recs;
 };
+
+static int gang_record(WarmGang *g, std::shared_ptr<GangRecord> *out) {
+  for (auto &r : g->recs)
+    if (r.use_count() == 1) {
+      *out = r;
+      return WDY_OK;
+    }
+  auto r = std::make_shared<GangRecord>();
+  r->device = g->device;
+  WDY_HIP(hipEventCreate(&r->start));
+  WDY_HIP(hipEventCreate(&r->done));
+  WDY_HIP(hipHostMalloc((void **)&r->h_err, 8 * WARM_GANG_MAX, hipHostMallocDefault));
+  WDY_HIP(hipMalloc(&r->d_err, 8 * WARM_GANG_MAX));
+  g->recs.push_back(r);
+  *out = r;
+  return WDY_OK;
+}
 
 int warm_gang_create(int device, WarmGang **out) {
   WarmGang *g = new (std::nothrow) WarmGang();
@@ -1686,6 +1747,7 @@ void warm_gang_destroy(WarmGang *g) {
   if (g->d_buf) (void)hipFree(g->d_buf);
   if (g->h_buf) (void)hipHostFree(g->h_buf);
   if (g->ev_up) (void)hipEventDestroy(g->ev_up);
+  g->recs.clear();  // records members still refer to live on with them
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }
@@ -1698,7 +1760,20 @@ bool warm_gang_eligible(wdy_sim *s, double h) {
 }
 
 void warm_wait(wdy_sim *s) {
-  if (s->warm && s->warm->pending) (void)hipEventSynchronize(s->warm->ev_err);
+  if (!s->warm || !s->warm->pending) return;
+  if (s->warm->rec) (void)hipEventSynchronize(s->warm->rec->done);
+  else (void)hipEventSynchronize(s->warm->ev_err);
+}
+
+void warm_gang_release(wdy_sim *s) {
+  if (s->warm && !s->warm->pending) s->warm->rec.reset();
+}
+
+bool warm_gang_events(wdy_sim *s, hipEvent_t *start, hipEvent_t *done) {
+  if (!s->warm || !s->warm->rec) return false;
+  *start = s->warm->rec->start;
+  *done = s->warm->rec->done;
+  return true;
 }
 
 // Enqueues one output interval of every member (nleap[i] steps of h[i]) on
@@ -1794,7 +1869,9 @@ int warm_gang_interval(WarmGang *g, wdy_sim *const *sims, int count,
   WDY_HIP(hipEventRecord(g->ev_up, g->stream));
   const WarmArgs *d_args = (const WarmArgs *)(g->d_buf + r_bytes);
   const u32 *d_bst = (const u32 *)(g->d_buf + r_bytes + a_bytes);
-  for (const Member &q : m) WDY_HIP(hipEventRecord(q.s->ev0, g->stream));
+  std::shared_ptr<GangRecord> rec;
+  if (int rc = gang_record(g, &rec)) return rc;
+  WDY_HIP(hipEventRecord(rec->start, g->stream));
   k_gang_reset<<<1, 256, 0, g->stream>>>((const GangReset *)g->d_buf,
                                          (u32)count, g->ring);
   WDY_CHECK_LAUNCH();
@@ -1807,12 +1884,16 @@ int warm_gang_interval(WarmGang *g, wdy_sim *const *sims, int count,
                                   g->ring + (e & 3u), g->ring + ((e + 2) & 3u)))
       return rc;
   }
+  k_gang_errs<<<1, 64, 0, g->stream>>>((const GangReset *)g->d_buf, (u32)count,
+                                       rec->d_err);
+  WDY_CHECK_LAUNCH();
+  WDY_HIP(hipMemcpyAsync(rec->h_err, rec->d_err, 8 * (size_t)count,
+                         hipMemcpyDeviceToHost, g->stream));
+  WDY_HIP(hipEventRecord(rec->done, g->stream));
   for (int i = 0; i < count; ++i) {
     wdy_sim *s = m[i].s;
-    WarmCtx &w = *s->warm;
-    WDY_HIP(hipMemcpyAsync(w.h_err, w.err, 8, hipMemcpyDeviceToHost, g->stream));
-    WDY_HIP(hipEventRecord(w.ev_err, g->stream));
-    WDY_HIP(hipEventRecord(s->ev1, g->stream));
+    s->warm->rec = rec;
+    s->warm->rec_idx = i;
     plan_end(s, m[i].p, h[i], nleap[i]);
   }
   return WDY_OK;

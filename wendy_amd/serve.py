@@ -10,10 +10,18 @@ min(tick, steps left) steps of size h (``tick`` for pool-resident requests,
 
 * Simulations with n <= ``split`` live in a ``Pool`` (wdy_pool_*): one device
   arena, one launch per tick for all of them, one workgroup per simulation.
-* Larger ones run as their own ``Sim`` (the whole-GPU large path), each
-  driven through all its intervals by a worker thread, so their launches
-  overlap the pool's on the device and their pace does not depend on the
-  pool's ticks.
+* Larger ones run as their own ``Sim`` (the whole-GPU large path).  By
+  default (``large='gang'``) every such simulation after its first output
+  interval advances in one ``Gang``: one kernel launch per leapfrog step for
+  all of them, on the gang's stream, driven by one host thread, so that many
+  medium simulations fill the GPU the way one large one does.  The first
+  interval (a full radix sort, which partitions the state for the warm
+  kernel) runs on the simulation's own stream in a preparation worker, and
+  finished simulations are read back and freed by a retire worker, so the
+  gang thread only ever waits for the gang's own work.  ``large='threads'``
+  drives each simulation through its intervals in its own worker thread
+  instead.  Either way the large path's pace does not depend on the pool's
+  ticks.
 
 Admission is first come, first served within each kind (pool-resident and
 large-path requests queue separately, so neither blocks the other): a request
@@ -25,13 +33,15 @@ A simulation's result does not depend on what else is being served.
 import collections
 import concurrent.futures
 import ctypes
+import queue
 import sys
+import threading
 import time
 
 import numpy
 
 from . import _lib
-from .nbody import Sim
+from .nbody import Gang, Sim
 
 # HBM bytes per sheet: the pool arena (two buffer sets of key, velocity,
 # index, mass) and an upper estimate for a large-path Sim (three state sets,
@@ -164,11 +174,19 @@ class _Big:
 
     def __init__(self, req, x, v, m):
         m = req.twopiG * numpy.asarray(m, dtype=numpy.float64)
+        # the whole-GPU path whatever n (a Sim would pick its one-workgroup
+        # path for n up to ~13,000 sheets, a long kernel on a queue the gang
+        # may share)
         self.sim = Sim(x, v, m, numpy.sum(m),
-                       -1. if req.omega is None else req.omega ** 2.)
+                       -1. if req.omega is None else req.omega ** 2.,
+                       path='large')
 
     def advance(self, h, k):
         self.sim.advance(h, k)
+
+    def settle(self):
+        """Wait for (and check) the interval in flight."""
+        self.sim.path_stats()
 
     def finish(self, n):
         x = numpy.empty(n)
@@ -177,6 +195,21 @@ class _Big:
         self.paths = self.sim.path_stats() + self.sim.warm_failures()
         self.sim.close()
         return x, v
+
+
+class _GangAdvance:
+    """The default gang: wendy_amd.Gang over the members' Sims."""
+
+    MAX = Gang.MAX
+
+    def __init__(self):
+        self.gang = Gang()
+
+    def advance(self, bigs, hs, ks):
+        self.gang.advance([b.sim for b in bigs], hs, ks)
+
+    def close(self):
+        self.gang.close()
 
 
 def _run_big(big, req, tick):
@@ -192,23 +225,206 @@ def _run_big(big, req, tick):
     return x, v, getattr(big, 'paths', None)
 
 
+class _LargeThreads:
+    """Large-path requests, one worker thread each."""
+
+    def __init__(self, big_factory, tick, workers):
+        self.big_factory = big_factory
+        self.tick = tick
+        self.pool = concurrent.futures.ThreadPoolExecutor(max(1, workers))
+        self.live = {}   # future -> result
+
+    def submit(self, r, res):
+        x, v, m = r.make()
+        b = self.big_factory(r, x, v, m)
+        self.live[self.pool.submit(_run_big, b, r, self.tick)] = res
+
+    def count(self):
+        return len(self.live)
+
+    def done(self, timeout=0.):
+        """Completed (result, (x, v, paths)) pairs; waits up to ``timeout``
+        seconds (None: until one completes) when none is ready."""
+        fs = [f for f in self.live if f.done()]
+        if not fs and self.live and (timeout is None or timeout > 0):
+            fs, _ = concurrent.futures.wait(
+                list(self.live), timeout=timeout,
+                return_when=concurrent.futures.FIRST_COMPLETED)
+        return [(self.live.pop(f), f.result()) for f in fs]
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+
+
+class _Member:
+    __slots__ = ('big', 'req', 'res', 'left')
+
+    def __init__(self, big, req, res, left):
+        self.big, self.req, self.res, self.left = big, req, res, left
+
+
+class _LargeGang:
+    """Large-path requests in gangs.  A preparation worker makes each
+    simulation and runs its first interval on its own stream; each of
+    ``ngangs`` threads advances the prepared simulations given to it
+    together (its own gang_factory() on its own stream, at most MAX members
+    per call, in arrival order), so that while one gang's step is bound by
+    a bucket's latency another's runs beside it; a retire worker reads back
+    and frees finished simulations.  A prepared simulation joins the gang
+    with the fewest sheets in flight."""
+
+    def __init__(self, big_factory, gang_factory, tick, prep_workers=4,
+                 ngangs=2):
+        self.big_factory = big_factory
+        self.tick = tick
+        self.gangs = [gang_factory() for _ in range(max(1, ngangs))]
+        self.max_members = getattr(self.gangs[0], 'MAX', 64)
+        self.prep = concurrent.futures.ThreadPoolExecutor(max(1, prep_workers))
+        self.retire = concurrent.futures.ThreadPoolExecutor(1)
+        self.ready = [queue.Queue() for _ in self.gangs]  # None: stop
+        self.load = [0] * len(self.gangs)   # sheets in flight per gang
+        self.lock = threading.Lock()
+        self.out = queue.Queue()        # (result, (x, v, paths) or exception)
+        self.n_live = 0                 # submitted, not yet returned by done()
+        self.gang_steps = 0             # gang calls (intervals) made
+        self.threads = [threading.Thread(target=self._loop, args=(i,), daemon=True)
+                        for i in range(len(self.gangs))]
+        for t in self.threads:
+            t.start()
+
+    def submit(self, r, res):
+        self.n_live += 1
+        self.prep.submit(self._prepare, r, res)
+
+    def count(self):
+        return self.n_live
+
+    def _finish(self, m, gi=None):
+        try:
+            x, v = m.big.finish(m.req.n)
+            self.out.put((m.res, (x, v, getattr(m.big, 'paths', None))))
+        except Exception as e:  # reported to the caller of done()
+            self.out.put((m.res, e))
+
+    def _prepare(self, r, res):
+        try:
+            x, v, m = r.make()
+            b = self.big_factory(r, x, v, m)
+            k = min(self.tick, r.steps)
+            b.advance(r.h, k)
+            settle = getattr(b, 'settle', None)
+            if settle is not None:
+                settle()
+            mem = _Member(b, r, res, r.steps - k)
+        except Exception as e:
+            self.out.put((res, e))
+            return
+        if mem.left == 0:
+            self._finish(mem)
+            return
+        with self.lock:
+            gi = min(range(len(self.gangs)), key=lambda i: self.load[i])
+            self.load[gi] += r.n
+        self.ready[gi].put(mem)
+
+    def _loop(self, gi):
+        gang, ready = self.gangs[gi], self.ready[gi]
+        active = []
+        stop = False
+        while True:
+            if not active:
+                if stop:
+                    return
+                item = ready.get()
+                if item is None:
+                    return
+                active.append(item)
+            while True:
+                try:
+                    item = ready.get_nowait()
+                except queue.Empty:
+                    break
+                if item is None:
+                    stop = True
+                else:
+                    active.append(item)
+            members = active[:self.max_members]
+            ks = [min(self.tick, m.left) for m in members]
+            try:
+                gang.advance([m.big for m in members],
+                             [m.req.h for m in members], ks)
+            except Exception as e:
+                for m in active:
+                    self.out.put((m.res, e))
+                active = []
+                continue
+            with self.lock:
+                self.gang_steps += 1
+            for m, k in zip(members, ks):
+                m.left -= k
+            for m in members:
+                if m.left == 0:
+                    with self.lock:
+                        self.load[gi] -= m.req.n
+                    self.retire.submit(self._finish, m)
+            active = [m for m in active if m.left > 0]
+
+    def done(self, timeout=0.):
+        got = []
+        try:
+            if timeout is None or timeout > 0:
+                if self.n_live:
+                    got.append(self.out.get(timeout=timeout))
+            while True:
+                got.append(self.out.get_nowait())
+        except queue.Empty:
+            pass
+        out = []
+        for res, val in got:
+            self.n_live -= 1
+            if isinstance(val, Exception):
+                raise val
+            out.append((res, val))
+        return out
+
+    def close(self):
+        for q in self.ready:
+            q.put(None)
+        for t in self.threads:
+            t.join()
+        self.prep.shutdown(wait=True)
+        self.retire.shutdown(wait=True)
+        for g in self.gangs:
+            close = getattr(g, 'close', None)
+            if close is not None:
+                close()
+
+
 class Server:
     """Continuous-batching scheduler.
 
     ``budget_bytes``: HBM the served state may use (the pool arena counts in
     full); ``pool_sheets``/``max_sims``: the pool's arena and slot count;
-    ``split``: largest n served from the pool; ``max_big``: large-path
-    simulations run at once; ``tick``: leapfrog steps per scheduling tick
-    (one output interval) of the pool; ``tick_large``: steps per output
-    interval of a large-path simulation (default ``tick``).
-    ``pool_factory``/``big_factory`` exist so the scheduling logic can be
-    exercised without a GPU.
+    ``split``: largest n served from the pool; ``large``: 'gang' (the large
+    path's simulations advance together, one launch per step) or 'threads'
+    (one worker thread each); ``ngangs``: gangs (streams and driving
+    threads) the large path spreads its simulations over; ``max_big``:
+    large-path simulations in flight at once (default 64 per gang, 8 with
+    threads); ``tick``: leapfrog
+    steps per scheduling tick (one output interval) of the pool;
+    ``tick_large``: steps per output interval of a large-path simulation
+    (default ``tick``).
+    ``pool_factory``/``big_factory``/``gang_factory`` exist so the
+    scheduling logic can be exercised without a GPU.
     """
 
     def __init__(self, budget_bytes=200e9, pool_sheets=None, max_sims=2048,
-                 split=1 << 16, max_big=8, tick=10, tick_large=None,
-                 pool_factory=None,
-                 big_factory=None, clock=time.perf_counter, sleep=time.sleep):
+                 split=1 << 16, max_big=None, tick=10, tick_large=None,
+                 large='gang', prep_workers=4, ngangs=2, pool_factory=None,
+                 big_factory=None, gang_factory=None,
+                 clock=time.perf_counter, sleep=time.sleep):
+        if large not in ('gang', 'threads'):
+            raise ValueError("large must be 'gang' or 'threads'")
         if pool_sheets is None:
             pool_sheets = max_sims * split
         self.pool_bytes = pool_sheets * POOL_BYTES_PER_SHEET
@@ -216,7 +432,12 @@ class Server:
             raise ValueError('the pool arena alone exceeds the HBM budget')
         self.budget = float(budget_bytes)
         self.split = int(split)
-        self.max_big = int(max_big)
+        self.large = large
+        self.max_big = int(max_big if max_big is not None
+                           else (64 * max(1, int(ngangs)) if large == 'gang'
+                                 else 8))
+        self.prep_workers = int(prep_workers)
+        self.ngangs = int(ngangs)
         self.tick = int(tick)
         self.tick_large = int(tick if tick_large is None else tick_large)
         self.clock = clock
@@ -224,6 +445,7 @@ class Server:
         make_pool = pool_factory or (lambda: Pool(pool_sheets, max_sims, split))
         self.pool = make_pool()
         self.big_factory = big_factory or _Big
+        self.gang_factory = gang_factory or _GangAdvance
         self.stats = {}
 
     def close(self):
@@ -240,13 +462,16 @@ class Server:
         qs = collections.deque()   # arrived, waiting for the pool
         qb = collections.deque()   # arrived, waiting to run as a Sim
         small = {}   # slot -> [request, steps left, result]
-        big = {}     # future -> result
         big_bytes = 0.
         results = {}
         ticks = stalls = 0
         work = 0
         pool_busy = 0.
-        workers = concurrent.futures.ThreadPoolExecutor(max(1, self.max_big))
+        if self.large == 'gang':
+            large = _LargeGang(self.big_factory, self.gang_factory,
+                               self.tick_large, self.prep_workers, self.ngangs)
+        else:
+            large = _LargeThreads(self.big_factory, self.tick_large, self.max_big)
 
         def arrive():
             t = now()
@@ -281,46 +506,39 @@ class Server:
             while qb:
                 r = qb[0]
                 need = r.n * SIM_BYTES_PER_SHEET
-                if (len(big) >= self.max_big
+                if (large.count() >= self.max_big
                         or self.pool_bytes + big_bytes + need > self.budget):
-                    if not big:
+                    if not large.count():
                         raise MemoryError('request %r does not fit the HBM budget'
                                           % (r.id,))
                     stalls += 1
                     break
-                res = result_for(r, 'sim')
-                x, v, m = r.make()
-                b = self.big_factory(r, x, v, m)
-                big[workers.submit(_run_big, b, r, self.tick_large)] = res
+                large.submit(r, result_for(r, 'sim'))
                 big_bytes += need
                 qb.popleft()
 
         def retire_big(done):
             nonlocal big_bytes, work
-            for f in done:
-                res = big.pop(f)
-                res.x, res.v, res.paths = f.result()
+            for res, (x, v, paths) in done:
+                res.x, res.v, res.paths = x, v, paths
                 res.completed = now()
                 results[res.id] = res
                 work += res.n * res.steps
                 big_bytes -= res.n * SIM_BYTES_PER_SHEET
 
         try:
-            while pending or qs or qb or small or big:
+            while pending or qs or qb or small or large.count():
                 arrive()
-                retire_big([f for f in list(big) if f.done()])
+                retire_big(large.done())
                 admit()
                 if not small:
-                    if not big:
+                    if not large.count():
                         if pending:
                             self.sleep(max(0., pending[0].arrival - now()))
                         continue
                     # only large-path work: wait for a completion or an arrival
                     wait = (max(0., pending[0].arrival - now()) if pending else None)
-                    done, _ = concurrent.futures.wait(
-                        list(big), timeout=wait,
-                        return_when=concurrent.futures.FIRST_COMPLETED)
-                    retire_big(done)
+                    retire_big(large.done(timeout=wait))
                     continue
                 ticks += 1
                 slots = list(small)
@@ -342,10 +560,12 @@ class Server:
                         work += res.n * res.steps
                         del small[s]
         finally:
-            workers.shutdown(wait=True)
+            large.close()
         wall = now()
         self.stats = {'ticks': ticks, 'admission_stalls': stalls, 'wall_s': wall,
                       'sheet_steps': work, 'pool_device_s': pool_busy}
+        if self.large == 'gang':
+            self.stats['gang_intervals'] = large.gang_steps
         return results
 
 
