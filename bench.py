@@ -240,7 +240,7 @@ def bench_serve(args, log):
     s = serve.Server(budget_bytes=200e9, max_sims=2048, split=SERVE_SPLIT,
                      tick=10, large='gang', ngangs=2)
     try:
-        res = s.run(reqs)
+        res = s.run(reqs, progress=lambda d: log('serve: progress %s' % d))
     finally:
         s.close()
     summ = serve.latency_summary(res)

@@ -109,13 +109,17 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
     u64 *__restrict__ kout, u32 *__restrict__ pout, u32 n, u32 shift,
     const u32 *__restrict__ offs, u64 *status, u32 *counter, u32 epoch,
     u32 spin_limit, u32 *err) {
-  __shared__ u32 s_tile;
+  // 54,272 bytes (106 allocation blocks of 512): three workgroups fit a
+  // CU's 160 KiB.  The tile number and the scan's wave totals live in the
+  // key staging area, which is written only after both are consumed; the
+  // digits' global bases replace the tile-local ones once the local
+  // scatter has read those.
   __shared__ u32 s_cnt[WAVES][RADIX];
   __shared__ u32 s_dbase[RADIX];
-  __shared__ u32 s_gbase[RADIX];
-  __shared__ u32 s_w[WAVES];
   __shared__ u64 s_key[TILE];
   __shared__ u32 s_pay[TILE];
+  u32 &s_tile = *(u32 *)&s_key[0];
+  u32 *const s_w = (u32 *)&s_key[1];
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) s_tile = atomicAdd(counter, 1u);
@@ -172,6 +176,7 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
   }
   const u32 dex = block_excl_scan(tot, s_w);
   s_dbase[t] = dex;
+  u32 gbase;
   {
     const u64 ep = (u64)(epoch & 0x3FFFFFFFu) << 34;
     u64 *mine = status + (u64)tile * RADIX + t;
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
       if (gave_up) atomicOr(err, 1u);
       atomic_store_agent(mine, ep | (2ull << 32) | (excl + tot));
     }
-    s_gbase[t] = offs[t] + excl - dex;
+    gbase = offs[t] + excl - dex;
   }
   __syncthreads();
 
@@ -220,10 +225,12 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
     }
   }
   __syncthreads();
+  s_dbase[t] = gbase;
+  __syncthreads();
   for (u32 i = t; i < tile_n; i += THREADS) {
     const u64 kk = s_key[i];
     const u32 d = (u32)(kk >> shift) & 255u;
-    const u32 g = s_gbase[d] + i;
+    const u32 g = s_dbase[d] + i;
     // always true unless a look-back wait of this or an earlier pass gave
     // up: its tile's offsets, and the digit counts of the keys a later pass
     // reads, are then wrong, and a write could land past the array

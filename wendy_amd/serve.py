@@ -453,11 +453,14 @@ class Server:
             self.pool.close()
         self.pool = None
 
-    def run(self, requests):
+    def run(self, requests, progress=None, progress_every=10.):
         """Serve every request; returns {id: Result}.  Latency counts from a
-        request's scheduled arrival to the host having its final state."""
+        request's scheduled arrival to the host having its final state.
+        ``progress`` (optional) is called about every ``progress_every``
+        seconds with a dict of the counts so far."""
         t0 = self.clock()
         now = lambda: self.clock() - t0
+        last_report = [0.]
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         qs = collections.deque()   # arrived, waiting for the pool
         qb = collections.deque()   # arrived, waiting to run as a Sim
@@ -526,8 +529,18 @@ class Server:
                 work += res.n * res.steps
                 big_bytes -= res.n * SIM_BYTES_PER_SHEET
 
+        def report():
+            t = now()
+            if progress is not None and t - last_report[0] >= progress_every:
+                last_report[0] = t
+                progress({'t_s': t, 'completed': len(results),
+                          'waiting': len(pending) + len(qs) + len(qb),
+                          'pool': len(small), 'large': large.count(),
+                          'sheet_steps': work})
+
         try:
             while pending or qs or qb or small or large.count():
+                report()
                 arrive()
                 retire_big(large.done())
                 admit()
