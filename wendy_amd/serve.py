@@ -271,16 +271,19 @@ class _LargeGang:
     per call, in arrival order), so that while one gang's step is bound by
     a bucket's latency another's runs beside it; a retire worker reads back
     and frees finished simulations.  A prepared simulation joins the gang
-    with the fewest sheets in flight."""
+    with the fewest sheets in flight.  Freeing a simulation's device memory
+    waits for the whole device (hipFree synchronises it), ~30 ms beside busy
+    gangs, so several retire workers free in parallel; with one, finished
+    simulations queued up and held back admission."""
 
     def __init__(self, big_factory, gang_factory, tick, prep_workers=4,
-                 ngangs=2):
+                 ngangs=2, retire_workers=8):
         self.big_factory = big_factory
         self.tick = tick
         self.gangs = [gang_factory() for _ in range(max(1, ngangs))]
         self.max_members = getattr(self.gangs[0], 'MAX', 64)
         self.prep = concurrent.futures.ThreadPoolExecutor(max(1, prep_workers))
-        self.retire = concurrent.futures.ThreadPoolExecutor(1)
+        self.retire = concurrent.futures.ThreadPoolExecutor(max(1, retire_workers))
         self.ready = [queue.Queue() for _ in self.gangs]  # None: stop
         self.load = [0] * len(self.gangs)   # sheets in flight per gang
         self.lock = threading.Lock()
@@ -408,7 +411,10 @@ class Server:
     ``split``: largest n served from the pool; ``large``: 'gang' (the large
     path's simulations advance together, one launch per step) or 'threads'
     (one worker thread each); ``ngangs``: gangs (streams and driving
-    threads) the large path spreads its simulations over; ``max_big``:
+    threads) the large path spreads its simulations over;
+    ``prep_workers``/``retire_workers``: threads that make and first-advance
+    large-path simulations, and that read back and free finished ones;
+    ``max_big``:
     large-path simulations in flight at once (default 64 per gang, 8 with
     threads); ``tick``: leapfrog
     steps per scheduling tick (one output interval) of the pool;
@@ -420,7 +426,8 @@ class Server:
 
     def __init__(self, budget_bytes=200e9, pool_sheets=None, max_sims=2048,
                  split=1 << 16, max_big=None, tick=10, tick_large=None,
-                 large='gang', prep_workers=4, ngangs=2, pool_factory=None,
+                 large='gang', prep_workers=4, ngangs=2, retire_workers=8,
+                 pool_factory=None,
                  big_factory=None, gang_factory=None,
                  clock=time.perf_counter, sleep=time.sleep):
         if large not in ('gang', 'threads'):
@@ -438,6 +445,7 @@ class Server:
                                  else 8))
         self.prep_workers = int(prep_workers)
         self.ngangs = int(ngangs)
+        self.retire_workers = int(retire_workers)
         self.tick = int(tick)
         self.tick_large = int(tick if tick_large is None else tick_large)
         self.clock = clock
@@ -472,7 +480,8 @@ class Server:
         pool_busy = 0.
         if self.large == 'gang':
             large = _LargeGang(self.big_factory, self.gang_factory,
-                               self.tick_large, self.prep_workers, self.ngangs)
+                               self.tick_large, self.prep_workers, self.ngangs,
+                               self.retire_workers)
         else:
             large = _LargeThreads(self.big_factory, self.tick_large, self.max_big)
 
