@@ -367,12 +367,13 @@ def copy_probe(mib=1024, launches=5):
                       'GB_per_s': 2 * nbytes / (ms.value * 1e-3) / 1e9}))
 
 
-def bench_plain(name, steps, warmup, log, e2e=True):
+def bench_plain(name, steps, warmup, log, e2e=True, warm=True):
     """One simulation of workload ``name`` on the GPU: ``warmup`` output
     intervals, then ``steps`` timed intervals of device work bracketed by
     synchronisations (the rate), a second timed region with an event pair
     around every launch (the dominant kernel's time per launch), and with
-    ``e2e`` the generator's loop in steady state."""
+    ``e2e`` the generator's loop in steady state.  ``warm=False`` keeps the
+    simulation on the radix path (a full sort every step)."""
     import wendy_amd
     from wendy_amd import _lib as wlib
     wl = WORKLOADS[name]
@@ -382,7 +383,8 @@ def bench_plain(name, steps, warmup, log, e2e=True):
     h = dt / nleap
     log('bench %s: n=%d nleap=%d steps=%d warmup=%d' % (name, n, nleap, steps,
                                                         warmup))
-    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.)
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.,
+                        warm=warm)
     del x, v, m
     for _ in range(warmup):
         sim.advance(h, nleap)
@@ -494,6 +496,9 @@ def main():
     ap.add_argument('--serve-window', type=float, default=3.)
     ap.add_argument('--serve-rate', type=float, default=200.)
     ap.add_argument('--no-unequal27', action='store_true')
+    ap.add_argument('--no-warm', action='store_true',
+                    help='radix path only (a full sort every step), for '
+                         'comparison and profiling; not the headline')
     ap.add_argument('--unequal27-steps', type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -506,11 +511,12 @@ def main():
         copy_probe()
         return
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    res = bench_plain(args.workload, args.steps, args.warmup, log, e2e=True)
+    res = bench_plain(args.workload, args.steps, args.warmup, log, e2e=True,
+                      warm=not args.no_warm)
     wl = WORKLOADS[args.workload]
     n, nleap, dt, omega = wl['n'], wl['nleap'], wl['dt'], wl['omega']
     rate, elapsed = res['value'], res['elapsed']
-    main_line = args.workload == 'plain24'
+    main_line = args.workload == 'plain24' and not args.no_warm
     ens = None if args.no_ensemble or not main_line else bench_ensemble(args, log)
     srv = None if args.no_serve or not main_line else bench_serve(args, log)
 
@@ -534,8 +540,8 @@ def main():
         'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic',
-        'config': {'workload': wl['label'], 'n': n, 'nleap': nleap, 'dt': dt,
-                   'omega': omega},
+        'config': {'workload': wl['label'] + (' [radix path only]' if args.no_warm else ''),
+                   'n': n, 'nleap': nleap, 'dt': dt, 'omega': omega},
         'e2e_value': res['e2e_value'],
         'kernel_ms': res['kernel_ms'],
         'paths': res['paths'],
