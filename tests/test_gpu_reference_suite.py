@@ -37,10 +37,10 @@ def _energy_conserved(x, v, m, dt, nleap, nout, omega=None, **kw):
 @pytest.mark.parametrize('masses', [[1., 1., 1.], [1., 2., 3.]])
 @pytest.mark.parametrize('omega', [None, 1.1])
 def test_energy_conservation_three_body(gpu_lib, masses, omega):
-    """tests/test_approx.py:6-32, tests/test_approx_harm.py:5-33 (nleap=1e5;
-    20 of the reference's 100 outputs to bound the run time)."""
+    """tests/test_approx.py:6-32, tests/test_approx_harm.py:5-33 (nleap=1e5,
+    all 100 outputs)."""
     _energy_conserved(numpy.array([-1.1, 0.1, 1.3]), numpy.array([3., 2., -5.]),
-                      numpy.array(masses), 0.05, 100000, 20, omega=omega)
+                      numpy.array(masses), 0.05, 100000, 100, omega=omega)
 
 
 @pytest.mark.parametrize('sort', ['quick', 'merge', 'tim', 'qsort', 'parallel'])
