@@ -50,6 +50,12 @@ extern "C" {
  * full radix sort when a bucket outgrows its limits.  This flag keeps the
  * large path on the radix sort alone (for comparison and testing). */
 #define WDY_NO_WARM 8
+/* The simulation will share the GPU with others, advanced in gangs
+ * (wdy_gang_advance): the large path keeps its throughput tile shape (1536-
+ * sheet buckets) at every size.  Without the flag a simulation below ~2^22
+ * sheets starts from larger workgroups, which shorten a step when its buckets
+ * alone do not fill the GPU. */
+#define WDY_SHARED_GPU 16
 
 typedef struct wdy_sim wdy_sim;
 

@@ -28,11 +28,11 @@ def ic(n, seed, mass):
                         else ('lognormal', 0.5))
 
 
-def make(spec):
+def make(spec, shared=True):
     n, seed, mass, omega, _, _, tune = spec
     x, v, m = ic(n, seed, mass)
     sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1. if omega is None else omega ** 2.,
-                        path='large')
+                        path='large', shared=shared)
     if tune:
         sim.tune(*tune)
     return sim
@@ -164,5 +164,31 @@ def test_gang_argument_errors(gpu_lib):
             read(s)
     finally:
         for s in sims:
+            s.close()
+        gang.close()
+
+
+def test_gang_members_on_other_shapes_run_alone(gpu_lib):
+    """Members whose tile shape has no gang form (a simulation made for a GPU
+    of its own starts from the shape fastest for its size) run their
+    intervals on their own streams: same results as alone, no gang steps
+    needed."""
+    specs = [MEMBERS[1], MEMBERS[2]]
+    gang = wendy_amd.Gang()
+    A = [make(s, shared=False) for s in specs] + [make(MEMBERS[0])]
+    B = [make(s, shared=False) for s in specs] + [make(MEMBERS[0])]
+    hs = [s[4] for s in specs] + [MEMBERS[0][4]]
+    ns = [s[5] for s in specs] + [MEMBERS[0][5]]
+    try:
+        for out in range(3):
+            gang.advance(A, hs, ns)
+            for b, h, k in zip(B, hs, ns):
+                b.advance(h, k)
+            for i, (a, b) in enumerate(zip(A, B)):
+                same(a, b, 'member %d output %d' % (i, out + 1))
+        for a, b in zip(A, B):
+            assert a.path_stats() == b.path_stats() and a.path_stats()[0] > 0
+    finally:
+        for s in A + B:
             s.close()
         gang.close()

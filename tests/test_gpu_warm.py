@@ -217,13 +217,14 @@ def test_first_warm_interval_after_fitting_succeeds(gpu_lib):
 @pytest.mark.parametrize('h,shape', [(0.001, 8), (0.06, 8), (0.15, 7), (0.4, 0)])
 def test_window_fitting_ladder(gpu_lib, h, shape):
     """The window fitted at the first interval from the sampled displacement
-    (N = 2^18): the default 1536-sheet buckets while the window stays within
-    48 buckets either side, then 2048-sheet buckets up to 60, then 4096-sheet
-    ones; in every case both intervals equal the oracle bit for bit (equal
-    masses 2^-18)."""
+    (N = 2^18, starting from the throughput shape a shared GPU uses): the
+    default 1536-sheet buckets while the window stays within 48 buckets
+    either side, then 2048-sheet buckets up to 60, then 4096-sheet ones; in
+    every case both intervals equal the oracle bit for bit (equal masses
+    2^-18)."""
     print('fitting ladder h', h, flush=True)
     x, v, m = O.sech2_disk(1 << 18, seed=3)
-    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large')
+    sim = wendy_amd.Sim(x, v, m, numpy.sum(m), -1., path='large', shared=True)
     st = O.ApproxState(x, v, m, 2 * h, 2, presorted_order=True)
     xo, vo = numpy.empty(len(x)), numpy.empty(len(x))
     try:
@@ -241,6 +242,33 @@ def test_window_fitting_ladder(gpu_lib, h, shape):
         assert sum(sim.path_stats()[:2]) == 4  # warm + radix steps
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize('lg,mass,shape', [
+    (14, 'equal', 3), (16, 'lognormal', 3), (18, 'equal', 1),
+    (20, 'equal', 5), (20, 'lognormal', 6), (21, 'lognormal', 6)])
+def test_initial_shape_by_size(gpu_lib, lg, mass, shape):
+    """A simulation that has the GPU to itself starts from the tile shape
+    measured fastest for its size (larger workgroups while its buckets do not
+    fill the GPU); a shared one (a gang member) from the default; both give
+    the oracle's trajectory."""
+    print('initial shape 2^%d %s' % (lg, mass), flush=True)
+    x, v, m = make(1 << lg, 40 + lg, mass)
+    exact = exact_for(m)
+    for shared, want in ((False, shape), (True, 8)):
+        sim = wendy_amd.Sim(x, v, m, numpy.sum(m), 0.1 ** 2., path='large',
+                            shared=shared)
+        st = O.ApproxState(x, v, m, 0.004, 4, omega=0.1)
+        xo, vo = numpy.empty(len(x)), numpy.empty(len(x))
+        try:
+            for k in range(2):
+                sim.advance(0.001, 4)
+                sim.read(xo, vo)
+                check((xo, vo), st.onestep(), exact, 'output %d' % (k + 1))
+            assert sim.warm_config()[0] == want, (shared, sim.warm_config())
+            assert sim.path_stats()[0] == 4   # the second interval ran warm
+        finally:
+            sim.close()
 
 
 def test_autotune_picks_a_shape_and_keeps_results(gpu_lib):

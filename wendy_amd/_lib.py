@@ -29,6 +29,7 @@ WDY_EQUAL_MASS = 1
 WDY_FORCE_LARGE = 2
 WDY_FORCE_SMALL = 4
 WDY_NO_WARM = 8
+WDY_SHARED_GPU = 16
 
 # every C symbol include/wendy_amd.h declares
 EXPORTED = ('wdy_abi_version', 'wdy_last_error', 'wdy_device_count',

@@ -183,7 +183,7 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     return fail(WDY_EINVAL, "wdy_create: n=%lld outside [1, 2^30)", (long long)n);
   if (!x_slot || !v_id || !m_id) return fail(WDY_EINVAL, "wdy_create: NULL array");
   if (flags & ~(WDY_EQUAL_MASS | WDY_FORCE_LARGE | WDY_FORCE_SMALL |
-                WDY_NO_WARM))
+                WDY_NO_WARM | WDY_SHARED_GPU))
     return fail(WDY_EINVAL, "wdy_create: unknown flags 0x%x", flags);
   if ((flags & WDY_FORCE_LARGE) && (flags & WDY_FORCE_SMALL))
     return fail(WDY_EINVAL, "wdy_create: WDY_FORCE_LARGE and WDY_FORCE_SMALL");
@@ -213,8 +213,10 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     if ((flags & WDY_FORCE_SMALL) && (u64)n > smax)
       return fail(WDY_EINVAL, "wdy_create: WDY_FORCE_SMALL but n=%lld > %u",
                   (long long)n, smax);
+    const u64 lim = eq ? SMALL_MAX_EQUAL : SMALL_MAX_UNEQUAL;
+    const u64 slim = smax < lim ? smax : lim;
     s->small = (flags & WDY_FORCE_SMALL) ||
-               (!(flags & WDY_FORCE_LARGE) && (u64)n <= smax);
+               (!(flags & WDY_FORCE_LARGE) && (u64)n <= slim);
     WDY_HIP(hipStreamCreate(&s->stream));
     WDY_HIP(hipStreamCreate(&s->copy_stream));
     WDY_HIP(hipEventCreate(&s->ev0));
@@ -274,7 +276,9 @@ static int create_impl(int64_t n, const double *x_slot, const i32 *id_slot,
     bool warm = !s->small && !(flags & WDY_NO_WARM);
     for (int64_t i = 0; warm && i < n; ++i) warm = m_id[i] >= 0.0;
     if (warm) {
-      r = warm_configure(s, WARM_SHAPE_DEFAULT, 0u, (u32)WARM_WINDOW_DEFAULT);
+      const int shape0 = (flags & WDY_SHARED_GPU) ? WARM_SHAPE_DEFAULT
+                                                  : warm_initial_shape(n, eq);
+      r = warm_configure(s, shape0, 0u, (u32)WARM_WINDOW_DEFAULT);
       if (r) return r;
       s->disp_rate = displacement_rate(n, x_slot, v_id, id_slot);
     }

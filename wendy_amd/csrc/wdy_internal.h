@@ -276,6 +276,15 @@ int energy_enqueue(hipStream_t st, u32 n, const double *x, const double *v,
                    u32 *epoch, const u32 **err_word);
 // small mode (wdy_small.hip)
 u32 small_max_n(int device);
+// Largest n the one-workgroup path takes by default (further capped by its
+// LDS capacity).  It runs a whole interval in one workgroup, so its time
+// grows with n while the large path's stays near its floor of ~0.33 ms per
+// 20-step interval up to ~2^16 sheets: measured, equal masses 0.21 vs 0.32
+// ms at 2^12 and 0.43 vs 0.32 at 2^13; unequal masses (whose mass prefix
+// the small path runs sequentially) 0.23 vs 0.40 at 2^10 and 0.43 vs 0.33
+// at 2^11.
+constexpr u64 SMALL_MAX_EQUAL = 6144;
+constexpr u64 SMALL_MAX_UNEQUAL = 1536;
 int launch_small_advance(wdy_sim *s, double h, int nleap, double *acc_out,
                          double *cum_out);
 int launch_energy(wdy_sim *s, double twopiG, double omega2, double *e_out);

@@ -32,6 +32,19 @@ constexpr int WARM_SHAPE_DEFAULT = 8;
 constexpr int WARM_SHAPE_WIDE = 7;
 constexpr int WARM_WINDOW_DEFAULT = 6;
 const WarmShape &warm_shape(int v);
+// The tile shape a simulation of n sheets starts from when it has the GPU
+// to itself.  While one step's buckets do not fill the GPU (N below ~2^22 at
+// the default's 1229-sheet buckets) a step lasts about as long as its
+// slowest bucket, and larger workgroups finish a bucket sooner.  Timed for
+// every shape on the sech^2 disk, equal and lognormal masses, one output of
+// 20 steps (per-interval ms, default -> best): 2^14-2^16 shape 3 (512
+// threads, 1024 sheets; 0.44 -> 0.34 equal, 0.49 -> 0.38 lognormal at
+// 2^14), 2^17-2^18 shape 1 (512 threads, 2048 sheets; 0.57 -> 0.51 at
+// 2^18), 2^19-2^20 shape 5 (512 threads, 3072 sheets) with equal masses and
+// shape 6 (768 threads, 3072 sheets) with unequal ones (1.46 -> 0.95 equal,
+// 1.57 -> 1.09 lognormal at 2^20), 2^21 shape 6 (within 3% of the best),
+// from 2^22 the default (every shape within 2%).
+int warm_initial_shape(int64_t n, bool equal_mass);
 inline u32 warm_default_target(int cap) {
   const u32 c = (u32)cap, room = c / 5u > 256u ? c / 5u : 256u;
   return c > 2u * room ? c - room : c / 2u;

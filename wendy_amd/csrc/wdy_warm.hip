@@ -1153,6 +1153,15 @@ static const WarmShape kShapes[WARM_NSHAPES] = {
 
 const WarmShape &warm_shape(int v) { return kShapes[v]; }
 
+int warm_initial_shape(int64_t n, bool equal_mass) {
+  const double lg = std::log2((double)(n > 1 ? n : 1));
+  if (lg < 16.5) return 3;
+  if (lg < 18.5) return 1;
+  if (lg < 20.5) return equal_mass ? 5 : 6;
+  if (lg < 21.5) return 6;
+  return WARM_SHAPE_DEFAULT;
+}
+
 int warm_step_launch(hipStream_t st, const WarmArgs &a, bool equal_mass,
                      bool harm, int mode, int shape) {
   switch (shape) {

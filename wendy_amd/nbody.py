@@ -27,7 +27,8 @@ class Gang:
     is a warm-start one, on the gang's own stream.  ``advance(sims, h,
     nleap)`` gives each member the result of ``sim.advance(h[i], nleap[i])``
     bit for bit; ``h`` and ``nleap`` may be scalars.  At most 64 members per
-    call."""
+    call.  Members made with ``Sim(..., shared=True)`` keep the tile shape
+    that suits many simulations sharing the GPU."""
 
     MAX = 64
 
@@ -73,10 +74,12 @@ class Sim:
     """
 
     def __init__(self, x, v, m, totmass, omega2, equal_mass=None, path=None,
-                 warm=True):
+                 warm=True, shared=False):
         """``path``: None (by size), 'small' (one persistent workgroup per
         output interval) or 'large' (whole-GPU steps).  ``warm=False`` keeps
-        the large path on the full radix sort every step."""
+        the large path on the full radix sort every step.  ``shared=True``:
+        the simulation will run in a ``Gang`` beside others, so it keeps the
+        throughput tile shape at every size (WDY_SHARED_GPU)."""
         self._h = None
         lib = _lib.lib()
         self._destroy = lib.wdy_destroy  # held for close() at interpreter exit
@@ -96,6 +99,8 @@ class Sim:
             raise ValueError("path must be None, 'small' or 'large'")
         if not warm:
             flags |= _lib.WDY_NO_WARM
+        if shared:
+            flags |= _lib.WDY_SHARED_GPU
         h = ctypes.c_void_p()
         _lib.check(lib.wdy_create(len(x), x, v, m, float(totmass),
                                   float(omega2), flags, ctypes.byref(h)))

@@ -175,11 +175,11 @@ class _Big:
     def __init__(self, req, x, v, m):
         m = req.twopiG * numpy.asarray(m, dtype=numpy.float64)
         # the whole-GPU path whatever n (a Sim would pick its one-workgroup
-        # path for n up to ~13,000 sheets, a long kernel on a queue the gang
-        # may share)
+        # path for the smallest n, a long kernel on a queue the gang may
+        # share), with the tile shape for a shared GPU
         self.sim = Sim(x, v, m, numpy.sum(m),
                        -1. if req.omega is None else req.omega ** 2.,
-                       path='large')
+                       path='large', shared=True)
 
     def advance(self, h, k):
         self.sim.advance(h, k)
