@@ -159,7 +159,6 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
   const u32 S = A.S, W = A.W, nb = A.nb;
-  const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
   WARM_STAMP(0);
   const u32 lo = j >= W ? j - W : 0u;
   const u32 hi = j + W < nb ? j + W : nb - 1;
@@ -403,9 +402,12 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
   }
 
   // ---- 7. partition by next-step bucket and write out ---------------------------
+  // one segment counter per destination; each sheet takes its slot with one
+  // LDS atomic (the order within a segment does not matter: the next step
+  // ranks by (position, index) and sums fixed-point masses)
   {
-    u32 *wc = s.bins;  // [NWAVE][S + 1] per-wave segment counters
-    for (u32 b = t; b < (u32)NWAVE * (S + 1); b += WT) wc[b] = 0;
+    u32 *wc = s.bins;  // [S + 1] segment counters, then starts
+    if (t <= S) wc[t] = 0;
     __syncthreads();
     u32 dd[PER], rr[PER];
 #pragma unroll
@@ -413,6 +415,7 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
       const u32 r = i * WT + t;
       const bool valid = r < c;
       u32 d = S;
+      rr[i] = 0;
       if (valid) {
         const u32 e = s.ord[r];
         u64 kp = s.key[e];
@@ -439,37 +442,16 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
             set_err(A.err, A.epoch, WARM_ERR_OVF);
           }
         }
+        rr[i] = atomicAdd(&wc[d], 1u);
       }
-      // wave-level ranking on d: one ballot per bit of d <= S
-      u64 peers = __ballot(valid);
-      for (int b = 0; b < dbits; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const u64 bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
-      }
-      const u32 below_l = (u32)__popcll(peers & ((1ull << lane) - 1ull));
-      const u32 old = valid ? wc[w * (S + 1) + d] : 0u;
-      if (valid && below_l == 0) wc[w * (S + 1) + d] = old + (u32)__popcll(peers);
       dd[i] = d;
-      rr[i] = old + below_l;
     }
     __syncthreads();
-    if (t <= S) {  // thread t owns segment t: per-wave prefixes and total
-      u32 tot = 0;
-#pragma unroll
-      for (int i = 0; i < NWAVE; ++i) {
-        const u32 x = wc[i * (S + 1) + t];
-        wc[i * (S + 1) + t] = tot;
-        tot += x;
-      }
-      s_segpre[t] = tot;
-    }
-    __syncthreads();
-    block_excl_u32_inplace<NWAVE>(s_segpre, S + 1, s_red);
+    block_excl_u32_inplace<NWAVE>(wc, S + 1, s_red);
     if (t <= S) {
-      const u32 nxt = t < S ? s_segpre[t + 1] : c;
-      A.wr.segoff[j * (S + 1) + t] = s_segpre[t];
-      A.wr.segcnt[j * (S + 1) + t] = nxt - s_segpre[t];
+      const u32 nxt = t < S ? wc[t + 1] : c;
+      A.wr.segoff[j * (S + 1) + t] = wc[t];
+      A.wr.segcnt[j * (S + 1) + t] = nxt - wc[t];
     }
     if (t == 0) A.wr.boff[j] = off;
 #pragma unroll
@@ -477,7 +459,7 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
       const u32 r = i * WT + t;
       if (r < c) {
         const u32 e = s.ord[r];
-        const u32 p = off + s_segpre[dd[i]] + wc[w * (S + 1) + dd[i]] + rr[i];
+        const u32 p = off + wc[dd[i]] + rr[i];
         A.kout[p] = s.key[e];
         A.vout[p] = s.v[e];
         A.iout[p] = s.id[e];
@@ -548,8 +530,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_gang(
 // LDS bytes of the pipelined kernel's tables for S segments, placed after
 // the bucket arrays: two splitter windows (the current bucket's, read by its
 // partition, and the next one's), then one segment table (S sources and
-// the segments' start offsets), the partition's S+1 output prefixes and
-// LOVF overflow indices.  The splitter windows (S+1 entries) and the start
+// the segments' start offsets) and LOVF overflow indices.  The splitter windows (S+1 entries) and the start
 // offsets (S) are padded to pow2_ge(S + 1) entries with the type's maximum,
 // so both are searched with fixed halving steps.  Only the splitter window
 // is needed by two buckets at once: the next bucket's segment table and
@@ -557,7 +538,7 @@ __global__ __launch_bounds__(SH::WT) void k_warm_gang(
 // bucket's partition, and again (over capacity) at its own commit, before
 // the following table overwrites them.
 __host__ __device__ constexpr size_t pipe_table_bytes(u32 S) {
-  return 16ull * pow2_ge(S + 1) + 4ull * (S + pow2_ge(S + 1) + (S + 1) + LOVF);
+  return 16ull * pow2_ge(S + 1) + 4ull * (S + pow2_ge(S + 1) + LOVF);
 }
 
 // Persistent, software-pipelined form of mode 0.  One workgroup per CU loops
@@ -582,8 +563,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
   u64 *const s_spl0 = lds_dyn + Lds<SH, EQ>::BYTES / 8;  // [2][TP]
   u32 *const s_segsrc = (u32 *)(s_spl0 + 2 * TP);        // [S]
   u32 *const s_segpre = s_segsrc + S;                    // [TP]
-  u32 *const s_segout = s_segpre + TP;                   // [S + 1]
-  u32 *const s_oidx = s_segout + S + 1;                  // [LOVF]
+  u32 *const s_oidx = s_segpre + TP;                     // [LOVF]
   auto s_spl = [&](int bb) { return s_spl0 + (u32)bb * TP; };
   const u32 PSEG = pow2_ge(S);  // the segment search's span
   __shared__ u32 s_novf;
@@ -594,7 +574,6 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
   __shared__ u64 s_u64[2];
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int dbits = 32 - __clz((int)S);  // bits of a segment index 0..S
   // padding of the start offsets (the scans write only [0, S))
   for (u32 q = S + t; q < TP; q += WT) s_segpre[q] = 0xffffffffu;
   if (t == 0) {
@@ -915,10 +894,14 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     WARM_STAMP(4);
     if (write) {
       // ---- partition by next-step bucket and write out ---------------------------
-      u32 *wc = s.bins;  // [NWAVE][S + 1] per-wave segment counters
-      u32 *segp = s_segout;
+      // segment counters [S + 1], one per workgroup: each sheet takes its
+      // slot in its segment with one LDS atomic.  The order of sheets within
+      // a segment does not matter (the next step ranks them by (position,
+      // index) and sums fixed-point masses), only that the slots are
+      // distinct.
+      u32 *wc = s.bins;
       const u64 *spl = s_spl(b);
-      for (u32 q = t; q < (u32)NWAVE * (S + 1); q += WT) wc[q] = 0;
+      if (t <= S) wc[t] = 0;
       __syncthreads();
       u32 dd[PER], rr[PER];
 #pragma unroll
@@ -926,6 +909,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         const u32 r = i * WT + t;
         const bool valid = r < c;
         u32 d = S;
+        rr[i] = 0;
         if (valid) {
           const u32 e = EQ ? r : (u32)s.ord[r];
           u64 kp = s.key[e];
@@ -950,36 +934,17 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
               set_err(A.err, A.epoch, WARM_ERR_OVF);
             }
           }
+          rr[i] = atomicAdd(&wc[d], 1u);
         }
-        u64 peers = __ballot(valid);
-        for (int bit = 0; bit < dbits; ++bit) {
-          const bool on = (d >> bit) & 1u;
-          const u64 bal = __ballot(on);
-          peers &= on ? bal : ~bal;
-        }
-        const u32 below_l = (u32)__popcll(peers & ((1ull << lane) - 1ull));
-        const u32 old = valid ? wc[w * (S + 1) + d] : 0u;
-        if (valid && below_l == 0) wc[w * (S + 1) + d] = old + (u32)__popcll(peers);
         dd[i] = d;
-        rr[i] = old + below_l;
       }
       __syncthreads();
+      // segment starts (exclusive scan of the counts), then the tables
+      (void)short_excl_u32_inplace<NWAVE>(wc, S + 1, s_red, &s_tot);
       if (t <= S) {
-        u32 tot = 0;
-#pragma unroll
-        for (int i = 0; i < NWAVE; ++i) {
-          const u32 x = wc[i * (S + 1) + t];
-          wc[i * (S + 1) + t] = tot;
-          tot += x;
-        }
-        segp[t] = tot;
-      }
-      if (S + 1 > 64) __syncthreads();
-      (void)short_excl_u32_inplace<NWAVE>(segp, S + 1, s_red, &s_tot);
-      if (t <= S) {
-        const u32 nxt = t < S ? segp[t + 1] : c;
-        A.wr.segoff[j * (S + 1) + t] = segp[t];
-        A.wr.segcnt[j * (S + 1) + t] = nxt - segp[t];
+        const u32 nxt = t < S ? wc[t + 1] : c;
+        A.wr.segoff[j * (S + 1) + t] = wc[t];
+        A.wr.segcnt[j * (S + 1) + t] = nxt - wc[t];
       }
       if (t == 0) A.wr.boff[j] = off;
 #pragma unroll
@@ -987,7 +952,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         const u32 r = i * WT + t;
         if (r < c) {
           const u32 e = EQ ? r : (u32)s.ord[r];
-          const u32 p = off + segp[dd[i]] + wc[w * (S + 1) + dd[i]] + rr[i];
+          const u32 p = off + wc[dd[i]] + rr[i];
           A.kout[p] = s.key[e];
           A.vout[p] = s.v[e];
           A.iout[p] = s.id[e];
