@@ -83,6 +83,11 @@ WORKLOADS = {
                     label=('plain single-simulation drop-in: N=2^24 equal-mass '
                            'sech^2 disk (z_h=1, sigma=1), omega=0.1, dt=0.005, '
                            'nleap=20; one step = one output interval')),
+    'unequal24': dict(n=1 << 24, nleap=20, dt=0.005, omega=0.1, lognormal=0.5,
+                      vshift=None,
+                      label=('unequal-mass disk: N=2^24, lognormal(0,0.5) masses '
+                             'normalised to 1, otherwise the plain workload '
+                             '(omega=0.1, dt=0.005, nleap=20)')),
     'unequal27': dict(n=1 << 27, nleap=10, dt=0.005, omega=None, lognormal=0.5,
                       vshift=0.3,
                       label=('large unequal-mass disk: N=2^27, lognormal(0,0.5) '

@@ -118,9 +118,13 @@ struct Lds {
 // rk[i] := the rank, in the (key, id) order of s[0..c), of sheet i * WT + t
 // (for i * WT + t < c).  s.ord holds the bins' contents on return; a barrier
 // must separate the ranking's LDS reads from later writes to s.
-template <class SH, bool EQ>
-__device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
-                                u64 *prof, u32 (&rk)[SH::PER]) {
+// KREG: the caller's registers kr[i], ir[i] hold the key and index of its
+// sheet i * WT + t (as stored in s), so they need not be read back from LDS.
+template <class SH, bool EQ, bool KREG>
+__device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
+                                     u64 *s_mm, u64 *prof, u32 (&rk)[SH::PER],
+                                     const u64 (&kr)[SH::PER],
+                                     const i32 (&ir)[SH::PER]) {
   constexpr int WT = SH::WT, PER = SH::PER, NQ = SH::NQ, NWAVE = SH::NWAVE;
   const u32 t = threadIdx.x;
   u64 kmin = ~0ull, kmax = 0ull;
@@ -128,7 +132,7 @@ __device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
   for (int i = 0; i < PER; ++i) {
     const u32 e = i * WT + t;
     if (e < c) {
-      const u64 k = s.key[e];
+      const u64 k = KREG ? kr[i] : s.key[e];
       kmin = k < kmin ? k : kmin;
       kmax = k > kmax ? k : kmax;
     }
@@ -171,7 +175,7 @@ __device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
     const u32 e = i * WT + t;
     q[i] = 0;
     if (e < c) {
-      const u64 k = s.key[e];
+      const u64 k = KREG ? kr[i] : s.key[e];
       u32 b;
       if (dbl) {
         const double qd = (decode_pos(k) - xmin) * scale;
@@ -242,8 +246,8 @@ __device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
       const u32 lo = q[i] == 0 ? 0u : bin_end(q[i] - 1), hi = bin_end(q[i]);
       u32 r = lo;
       if (hi - lo > 1) {
-        const u64 k = s.key[e];
-        const i32 id = s.id[e];
+        const u64 k = KREG ? kr[i] : s.key[e];
+        const i32 id = KREG ? ir[i] : s.id[e];
         u32 a = lo;
         for (; a + 2 <= hi; a += 2) {  // two at a time: the LDS loads overlap
           const unsigned short f0 = s.ord[a], f1 = s.ord[a + 1];
@@ -261,14 +265,32 @@ __device__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
   }
 }
 
-// s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
 template <class SH, bool EQ>
-__device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
-                                u64 *prof) {
+__device__ __forceinline__ void lds_bucket_rank(Lds<SH, EQ> &s, u32 c, u64 *s_red,
+                                                u64 *s_mm, u64 *prof,
+                                                u32 (&rk)[SH::PER]) {
+  u64 kr[SH::PER];
+  i32 ir[SH::PER];
+  lds_bucket_rank_impl<SH, EQ, false>(s, c, s_red, s_mm, prof, rk, kr, ir);
+}
+
+template <class SH, bool EQ>
+__device__ __forceinline__ void lds_bucket_rank_regs(
+    Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm, u64 *prof,
+    u32 (&rk)[SH::PER], const u64 (&kr)[SH::PER], const i32 (&ir)[SH::PER]) {
+  lds_bucket_rank_impl<SH, EQ, true>(s, c, s_red, s_mm, prof, rk, kr, ir);
+}
+
+// s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.
+template <class SH, bool EQ, bool KREG>
+__device__ void lds_bucket_sort_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
+                                     u64 *s_mm, u64 *prof,
+                                     const u64 (&kr)[SH::PER],
+                                     const i32 (&ir)[SH::PER]) {
   constexpr int WT = SH::WT, PER = SH::PER;
   const u32 t = threadIdx.x;
   u32 rk[PER];
-  lds_bucket_rank<SH, EQ>(s, c, s_red, s_mm, prof, rk);
+  lds_bucket_rank_impl<SH, EQ, KREG>(s, c, s_red, s_mm, prof, rk, kr, ir);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
@@ -276,6 +298,23 @@ __device__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm,
     if (e < c) s.ord[rk[i]] = (unsigned short)e;
   }
   __syncthreads();
+}
+
+template <class SH, bool EQ>
+__device__ __forceinline__ void lds_bucket_sort(Lds<SH, EQ> &s, u32 c, u64 *s_red,
+                                                u64 *s_mm, u64 *prof) {
+  u64 kr[SH::PER];
+  i32 ir[SH::PER];
+  lds_bucket_sort_impl<SH, EQ, false>(s, c, s_red, s_mm, prof, kr, ir);
+}
+
+// The same, with the caller's sheets' keys and indices in registers (see
+// lds_bucket_rank_impl).
+template <class SH, bool EQ>
+__device__ __forceinline__ void lds_bucket_sort_regs(
+    Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm, u64 *prof,
+    const u64 (&kr)[SH::PER], const i32 (&ir)[SH::PER]) {
+  lds_bucket_sort_impl<SH, EQ, true>(s, c, s_red, s_mm, prof, kr, ir);
 }
 
 

@@ -685,6 +685,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         if (e < c) {
           u64 kk = dk[i];
           if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * dv[i]);
+          dk[i] = kk;  // the ranking and the force read it from here
           s.key[e] = kk;
           s.v[e] = dv[i];
           s.id[e] = di[i];
@@ -725,11 +726,12 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     u32 rk[PER];
     if (ok) {
       if constexpr (EQ)
-        lds_bucket_rank<SH, EQ>(s, c, s_red, s_mm,
-                                A.prof ? A.prof + 8ull * j : nullptr, rk);
+        lds_bucket_rank_regs<SH, EQ>(s, c, s_red, s_mm,
+                                     A.prof ? A.prof + 8ull * j : nullptr, rk,
+                                     dk, di);
       else
-        lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm,
-                                A.prof ? A.prof + 8ull * j : nullptr);
+        lds_bucket_sort_regs<SH, EQ>(s, c, s_red, s_mm,
+                                     A.prof ? A.prof + 8ull * j : nullptr, dk, di);
     } else {
       __syncthreads();
     }
@@ -830,12 +832,12 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         const u32 e = i * WT + t;
         if (e < c) {
           const u32 R = off + rk[i];
-          const u64 kold = s.key[e];
+          const u64 kold = dk[i];  // this thread's sheet: still in registers
           double x = decode_pos(kold);
           double F = (A.totmass - 2.0 * ((double)R * A.mq)) - A.mq;
           if (HARM) F = F - A.omega2 * x;
           const double a = (EXT ? ext_accel(A.ext, x, A.t) : 0.0) + F;
-          const double v = s.v[e] + A.h * a;
+          const double v = dv[i] + A.h * a;
           x = x + A.hd * v;
           s.v[e] = v;
           s.key[e] = encode_pos(x);
