@@ -35,6 +35,26 @@ __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
   return wp + x - v;
 }
 
+// The same for u32 values whose total fits 32 bits.
+template <int NWAVE>
+__device__ u32 block_excl_u32(u32 v, u64 *s_red) {
+  const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  u32 x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 y = __shfl_up(x, o, 64);
+    if (lane >= (u32)o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) s_red[w] = x;
+  __syncthreads();
+  u32 wp = 0;
+#pragma unroll
+  for (int i = 0; i < NWAVE; ++i)
+    if (i < (int)w) wp += (u32)s_red[i];
+  return wp + x - v;
+}
+
 // Exclusive scan of s[0..n) (n <= threads) in place; returns the total.
 template <int NWAVE>
 __device__ u32 block_excl_u32_inplace(u32 *s, u32 n, u64 *s_red) {
@@ -195,22 +215,22 @@ __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
   {
     constexpr int WPT = SH::NQW / WT;  // bin words t*WPT .. t*WPT + WPT - 1
     u32 loc[WPT];
-    u64 sum = 0;
+    u32 sum = 0;  // counts total at most CAP
 #pragma unroll
     for (int i = 0; i < WPT; ++i) {
       loc[i] = s.bins[t * WPT + i];
       sum += SH::PACK ? (loc[i] & 0xffffu) + (loc[i] >> 16) : loc[i];
     }
-    u64 run = block_excl_u64<NWAVE>(sum, s_red, nullptr);
+    u32 run = block_excl_u32<NWAVE>(sum, s_red);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < WPT; ++i) {
       if (SH::PACK) {  // exclusive offsets of both bins of the word
-        const u32 lo = (u32)run, hi = lo + (loc[i] & 0xffffu);
+        const u32 lo = run, hi = lo + (loc[i] & 0xffffu);
         s.bins[t * WPT + i] = lo | (hi << 16);
         run += (loc[i] & 0xffffu) + (loc[i] >> 16);
       } else {
-        s.bins[t * WPT + i] = (u32)run;
+        s.bins[t * WPT + i] = run;
         run += loc[i];
       }
     }

@@ -687,7 +687,9 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * dv[i]);
           dk[i] = kk;  // the ranking and the force read it from here
           s.key[e] = kk;
-          s.v[e] = dv[i];
+          // equal masses: only this thread's force reads its velocities,
+          // from registers, and it stores the kicked ones here itself
+          if (!EQ) s.v[e] = dv[i];
           s.id[e] = di[i];
           if (!EQ) {
             s.m[e] = dm[i];
