@@ -226,17 +226,19 @@ def serve_workload(window_s, rate=200., seed=0, pool_log2=21):
 
 
 # largest request the serving pool holds (one workgroup per simulation);
-# larger ones advance in gangs on the whole-GPU path.  Measured over a 1 s
-# window of the serving load: split 2^13 served 2.2e10 sheet-steps/s, 2^12
-# 2.0e10, 2^16 (the pool then bounded by its largest, one-workgroup members)
-# 0.97e10.
-SERVE_SPLIT = 1 << 13
+# larger ones advance in gangs on the whole-GPU path.  Measured over a 3 s
+# window of the serving load: split 2^14 served 2.87e10 sheet-steps/s (p50
+# 1.4 s), 2^13 2.82e10 (p50 2.3 s), 2^15 2.78e10; over a 1 s window 2^16
+# (the pool then bounded by its largest, one-workgroup members) served
+# about half as much.
+SERVE_SPLIT = 1 << 14
 
 
 def bench_serve(args, log):
     """Continuous batching of the serving load on one GPU (wendy_amd.serve):
-    a 200 GB HBM budget, pool of 2048 slots for N <= 2^13, larger requests on
-    the large path in two gangs (up to 128 at once), intervals of 10 steps."""
+    a 200 GB HBM budget, pool of 2048 slots for N <= SERVE_SPLIT, larger
+    requests on the large path in two gangs (up to 128 at once), intervals of
+    10 steps."""
     from wendy_amd import serve
     reqs = serve_workload(args.serve_window, args.serve_rate)
     offered = sum(r.n * r.steps for r in reqs) / args.serve_window
@@ -256,8 +258,9 @@ def bench_serve(args, log):
                          '[2^10, 2^20], lognormal(0,0.3) masses, z_h~U(0.5,2), '
                          'sigma=sqrt(z_h), omega~U(0,0.2), 100-10000 steps '
                          'log-uniform, h=0.01, 200 GB HBM budget; pool for '
-                         'N <= 2^13, larger N in two gangs'
-                         % (args.serve_rate, args.serve_window)),
+                         'N <= 2^%d, larger N in two gangs'
+                         % (args.serve_rate, args.serve_window,
+                            SERVE_SPLIT.bit_length() - 1)),
             'requests': len(reqs), 'offered_sheet_steps_per_s': offered,
             'p50_latency_s': summ['p50_s'], 'p99_latency_s': summ['p99_s'],
             'mean_latency_s': summ['mean_s'],

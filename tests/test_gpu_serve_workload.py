@@ -3,7 +3,7 @@
 lognormal(0, 0.3) masses; omega ~ U(0, 0.2); h = 0.01), drawn from the same
 generator the benchmark uses (bench.serve_workload), over a short arrival
 window and with the step counts capped so the run stays short; both kinds of
-request are present: pool-resident (n <= 2^13) and large-path (up to 2^20,
+request are present: pool-resident (n <= 2^14) and large-path (up to 2^20,
 advancing together in gangs beside the pool's launches).
 
 Each result is compared bit for bit with the same request served alone (a

@@ -192,6 +192,8 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
 
     // ---- 3. gather: positions first, then every load of the thread --------
     u64 msum = 0;
+    u64 kreg[PER];  // this thread's keys and indices, for the ranking
+    i32 ireg[PER];
     if (ok) {
       u32 pos[PER];
 #pragma unroll
@@ -224,6 +226,8 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
         if (e < c) {
           u64 kk = k[i];
           if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * v[i]);
+          kreg[i] = kk;
+          ireg[i] = id[i];
           s.key[e] = kk;
           s.v[e] = v[i];
           s.id[e] = id[i];
@@ -252,8 +256,9 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
     WARM_STAMP(1);
 
     // ---- 4. sort the bucket in LDS, while the predecessors publish ---------
-    if (ok) lds_bucket_sort<SH, EQ>(s, c, s_red, s_mm,
-                                    A.prof ? A.prof + 8ull * j : nullptr);
+    if (ok) lds_bucket_sort_regs<SH, EQ>(s, c, s_red, s_mm,
+                                         A.prof ? A.prof + 8ull * j : nullptr,
+                                         kreg, ireg);
     else __syncthreads();
     WARM_STAMP(2);
 
