@@ -12,16 +12,68 @@ __device__ __forceinline__ u64 mass_fix(double m, double fscale) {
   return (u64)llrint(m * fscale);
 }
 
+// Wave-wide scans and reductions on DPP lane moves (row shifts within rows
+// of 16 lanes, then the broadcasts of lane 15 and lane 31 into the rows
+// above): no LDS traffic, a few cycles per step.  Every lane of the wave
+// must be active.  A move whose source lane is outside the row, or whose
+// row is masked off, returns `old` (the operation's identity).
+constexpr int DPP_SHR1 = 0x111, DPP_SHR2 = 0x112, DPP_SHR4 = 0x114,
+              DPP_SHR8 = 0x118, DPP_BCAST15 = 0x142, DPP_BCAST31 = 0x143;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ u32 dpp_mov(u32 old, u32 x) {
+  return (u32)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ u64 dpp_mov(u64 old, u64 x) {
+  return ((u64)dpp_mov<CTRL, ROWS>((u32)(old >> 32), (u32)(x >> 32)) << 32) |
+         dpp_mov<CTRL, ROWS>((u32)old, (u32)x);
+}
+// inclusive prefix sum over the lanes (lane 63: the wave's total)
+template <class T>
+__device__ __forceinline__ T wave_incl_add(T x) {
+  x += dpp_mov<DPP_SHR1, 0xf>((T)0, x);
+  x += dpp_mov<DPP_SHR2, 0xf>((T)0, x);
+  x += dpp_mov<DPP_SHR4, 0xf>((T)0, x);
+  x += dpp_mov<DPP_SHR8, 0xf>((T)0, x);
+  x += dpp_mov<DPP_BCAST15, 0xa>((T)0, x);
+  x += dpp_mov<DPP_BCAST31, 0xc>((T)0, x);
+  return x;
+}
+// lane 63's value, in every lane
+__device__ __forceinline__ u32 wave_last(u32 x) {
+  return (u32)__builtin_amdgcn_readlane((int)x, 63);
+}
+__device__ __forceinline__ u64 wave_last(u64 x) {
+  return ((u64)wave_last((u32)(x >> 32)) << 32) | wave_last((u32)x);
+}
+// the wave's minimum and maximum, in every lane
+__device__ __forceinline__ u64 wave_min(u64 x) {
+  u64 y;
+  y = dpp_mov<DPP_SHR1, 0xf>(~0ull, x); x = y < x ? y : x;
+  y = dpp_mov<DPP_SHR2, 0xf>(~0ull, x); x = y < x ? y : x;
+  y = dpp_mov<DPP_SHR4, 0xf>(~0ull, x); x = y < x ? y : x;
+  y = dpp_mov<DPP_SHR8, 0xf>(~0ull, x); x = y < x ? y : x;
+  y = dpp_mov<DPP_BCAST15, 0xa>(~0ull, x); x = y < x ? y : x;
+  y = dpp_mov<DPP_BCAST31, 0xc>(~0ull, x); x = y < x ? y : x;
+  return wave_last(x);
+}
+__device__ __forceinline__ u64 wave_max(u64 x) {
+  u64 y;
+  y = dpp_mov<DPP_SHR1, 0xf>(0ull, x); x = y > x ? y : x;
+  y = dpp_mov<DPP_SHR2, 0xf>(0ull, x); x = y > x ? y : x;
+  y = dpp_mov<DPP_SHR4, 0xf>(0ull, x); x = y > x ? y : x;
+  y = dpp_mov<DPP_SHR8, 0xf>(0ull, x); x = y > x ? y : x;
+  y = dpp_mov<DPP_BCAST15, 0xa>(0ull, x); x = y > x ? y : x;
+  y = dpp_mov<DPP_BCAST31, 0xc>(0ull, x); x = y > x ? y : x;
+  return wave_last(x);
+}
+
 // Exclusive scan of one u64 per thread over the block; *total gets the sum.
 template <int NWAVE>
 __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
   const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  u64 x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const u64 y = __shfl_up(x, o, 64);
-    if (lane >= (u32)o) x += y;
-  }
+  const u64 x = wave_incl_add(v);
   __syncthreads();
   if (lane == 63) s_red[w] = x;
   __syncthreads();
@@ -39,12 +91,7 @@ __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
 template <int NWAVE>
 __device__ u32 block_excl_u32(u32 v, u64 *s_red) {
   const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  u32 x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const u32 y = __shfl_up(x, o, 64);
-    if (lane >= (u32)o) x += y;
-  }
+  const u32 x = wave_incl_add(v);
   __syncthreads();
   if (lane == 63) s_red[w] = x;
   __syncthreads();
@@ -78,12 +125,7 @@ __device__ u32 short_excl_u32_inplace(u32 *s, u32 n, u64 *s_red, u32 *s_tot) {
   const u32 t = threadIdx.x;
   if (t < 64) {
     const u32 v = t < n ? s[t] : 0u;
-    u32 x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const u32 y = __shfl_up(x, o, 64);
-      if (t >= (u32)o) x += y;
-    }
+    const u32 x = wave_incl_add(v);
     if (t < n) s[t] = x - v;
     if (t == 63) *s_tot = x;
   }
@@ -157,12 +199,8 @@ __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
       kmax = k > kmax ? k : kmax;
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const u64 a = __shfl_down(kmin, o, 64), b = __shfl_down(kmax, o, 64);
-    kmin = a < kmin ? a : kmin;
-    kmax = b > kmax ? b : kmax;
-  }
+  kmin = wave_min(kmin);
+  kmax = wave_max(kmax);
   if ((t & 63) == 0) {
     s_mm[2 * (t >> 6)] = kmin;
     s_mm[2 * (t >> 6) + 1] = kmax;

@@ -307,13 +307,8 @@ __device__ __forceinline__ void warm_bucket(const WarmArgs &A, const u32 j,
             cc = (u32)wd;
             mm = mv;
           }
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) {
-            cc += __shfl_down(cc, o, 64);
-            mm += __shfl_down(mm, o, 64);
-          }
-          ecnt += __shfl(cc, 0, 64);
-          emass += __shfl(mm, 0, 64);
+          ecnt += wave_last(wave_incl_add(cc));
+          if (!EQ) emass += wave_last(wave_incl_add(mm));
           if (first < 64) break;
           base -= 64;
         }
@@ -784,13 +779,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
             cc = (u32)wd;
             mm = mv;
           }
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) {
-            cc += __shfl_down(cc, o, 64);
-            mm += __shfl_down(mm, o, 64);
-          }
-          ecnt += __shfl(cc, 0, 64);
-          emass += __shfl(mm, 0, 64);
+          ecnt += wave_last(wave_incl_add(cc));
+          if (!EQ) emass += wave_last(wave_incl_add(mm));
           if (first < 64) break;
           base -= 64;
         }
