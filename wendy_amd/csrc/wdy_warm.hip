@@ -552,6 +552,24 @@ __host__ __device__ constexpr size_t pipe_table_bytes(u32 S) {
 // buckets, so the chain always advances.
 // EXT: the simulation has an external force (A.ext); without one the kernel
 // carries none of its code, which keeps the register count down.
+// A field of a kernel's argument block, loaded where it is used: the empty
+// asm statement keeps the load from being hoisted to the kernel's entry, so
+// a field that only rare paths read (overflow lists, error word, profile
+// buffer) does not hold scalar registers for the whole kernel.  For a kernel
+// whose only argument is a WarmArgs (it starts at offset 0 of the block).
+typedef __attribute__((address_space(4))) const char karg_char;
+template <class T>
+__device__ __forceinline__ T karg_load(size_t off) {
+  karg_char *p = (karg_char *)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *(__attribute__((address_space(4))) const T *)(p + off);
+}
+#define KARG(f) karg_load<decltype(A.f)>(offsetof(WarmArgs, f))
+#define PIPE_STAMP(k)                                                  \
+  do {                                                                 \
+    if (prof_on && t == 0) KARG(prof)[8ull * j + (k)] = wall_clock64(); \
+  } while (0)
+
 template <class SH, bool EQ, bool HARM, bool EXT>
 __global__ __launch_bounds__(SH::WT)
 __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmArgs A) {
@@ -574,11 +592,12 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
   __shared__ u64 s_u64[2];
 
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool prof_on = A.prof != nullptr;
   // padding of the start offsets (the scans write only [0, S))
   for (u32 q = S + t; q < TP; q += WT) s_segpre[q] = 0xffffffffu;
   if (t == 0) {
     s_u[0] = atomicAdd(A.tick, 1u);
-    s_u[1] = ld_u32(A.err);
+    s_u[1] = ld_u32(KARG(err));
     s_novf = 0;
   }
   const u32 novf_in = ld_u32(A.ovf_r);
@@ -620,7 +639,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     if (t <= hi - lo + 1) s_spl(bb)[t] = tb_spl;
     else if (t < TP) s_spl(bb)[t] = ~0ull;  // padding past the window
     for (u32 q2 = t; q2 < novf_all; q2 += WT)
-      if (A.rd.odst[q2] == q) {
+      if (KARG(rd.odst)[q2] == q) {
         const u32 slot = atomicAdd(&s_novf, 1u);
         if (slot < (u32)LOVF) s_oidx[slot] = q2;
       }
@@ -650,10 +669,10 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         if (!EQ) dm[i] = A.min[pos];
       } else if (e < c) {
         const u32 q = s_oidx[e - nseg];
-        dk[i] = A.rd.okey[q];
-        dv[i] = A.rd.ov[q];
-        di[i] = A.rd.oid[q];
-        if (!EQ) dm[i] = A.rd.om[q];
+        dk[i] = KARG(rd.okey)[q];
+        dv[i] = KARG(rd.ov)[q];
+        di[i] = KARG(rd.oid)[q];
+        if (!EQ) dm[i] = KARG(rd.om)[q];
       }
     }
   };
@@ -675,7 +694,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
   while (j < nb) {
     const u32 lo = j >= W ? j - W : 0u, hi = j + W < nb ? j + W : nb - 1;
     // ---- commit bucket j to LDS; publish its aggregate ----------------------
-    WARM_STAMP(0);
+    PIPE_STAMP(0);
     if (t == 0) s_u[0] = atomicAdd(A.tick, 1u);  // the next ticket
     u64 msum = 0;
     if (ok) {
@@ -702,7 +721,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       for (u32 e = t; e < nseg; e += WT)
         msum += mass_fix(A.min[seg_pos(s_segpre, s_segsrc, S, e)], A.fscale);
       for (u32 q2 = t; q2 < novf_all; q2 += WT)
-        if (A.rd.odst[q2] == j) msum += mass_fix(A.rd.om[q2], A.fscale);
+        if (KARG(rd.odst)[q2] == j) msum += mass_fix(KARG(rd.om)[q2], A.fscale);
     }
     u64 mtot = 0;
     if (!EQ) (void)block_excl_u64<NWAVE>(msum, s_red, &mtot);
@@ -710,11 +729,11 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       // (equal masses published the count with the table)
       if (!EQ) publish(A.status + STATUS_WORDS * j, ep, j == 0 ? 2u : 1u, c, mtot, true);
       if (!ok)
-        set_err(A.err, A.epoch,
+        set_err(KARG(err), A.epoch,
                 c - nseg > (u32)LOVF ? WARM_ERR_LOVF : WARM_ERR_CAPACITY);
     }
     __syncthreads();
-    WARM_STAMP(1);
+    PIPE_STAMP(1);
     const u32 jn = s_u[0];
     const int bn = b ^ 1;
     if (t == 0) s_novf = 0;
@@ -729,15 +748,15 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     if (ok) {
       if constexpr (EQ)
         lds_bucket_rank_regs<SH, EQ>(s, c, s_red, s_mm,
-                                     A.prof ? A.prof + 8ull * j : nullptr, rk,
+                                     prof_on ? KARG(prof) + 8ull * j : nullptr, rk,
                                      dk, di);
       else
         lds_bucket_sort_regs<SH, EQ>(s, c, s_red, s_mm,
-                                     A.prof ? A.prof + 8ull * j : nullptr, dk, di);
+                                     prof_on ? KARG(prof) + 8ull * j : nullptr, dk, di);
     } else {
       __syncthreads();
     }
-    WARM_STAMP(2);
+    PIPE_STAMP(2);
     if (w == 0) {
       u64 ecnt = 0, emass = 0;
       bool stalled = false;
@@ -786,7 +805,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         }
         if (lane == 0) {
           if (stalled) {
-            set_err(A.err, A.epoch, WARM_ERR_STALL);
+            set_err(KARG(err), A.epoch, WARM_ERR_STALL);
           } else {
             publish(A.status + STATUS_WORDS * j, ep, 2u, (u32)(ecnt + c),
                     emass + mtot, !EQ);
@@ -800,7 +819,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       }
     }
     __syncthreads();
-    WARM_STAMP(3);
+    PIPE_STAMP(3);
     const u32 off = (u32)s_u64[0];
     const u64 bfix = s_u64[1];
     const bool write = ok && !s_u[2];
@@ -890,7 +909,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     }
     // ---- the next bucket's loads, in flight during the partition --------------------
     if (ok_n) data_issue(nseg_n, c_n);
-    WARM_STAMP(4);
+    PIPE_STAMP(4);
     if (write) {
       // ---- partition by next-step bucket and write out ---------------------------
       // segment counters [S + 1], one per workgroup: each sheet takes its
@@ -922,15 +941,15 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
             d = lo + a + W - j;
           } else {
             const u32 dst = bucket_of(A.rd.spl, nb, kp);
-            const u32 q = atomicAdd(A.ovf_w, 1u);
+            const u32 q = atomicAdd(KARG(ovf_w), 1u);
             if (q < WARM_OVF_CAP) {
-              A.wr.okey[q] = s.key[e];
-              A.wr.ov[q] = s.v[e];
-              A.wr.oid[q] = s.id[e];
-              A.wr.om[q] = EQ ? A.mq : s.m[e];
-              A.wr.odst[q] = dst;
+              KARG(wr.okey)[q] = s.key[e];
+              KARG(wr.ov)[q] = s.v[e];
+              KARG(wr.oid)[q] = s.id[e];
+              KARG(wr.om)[q] = EQ ? A.mq : s.m[e];
+              KARG(wr.odst)[q] = dst;
             } else {
-              set_err(A.err, A.epoch, WARM_ERR_OVF);
+              set_err(KARG(err), A.epoch, WARM_ERR_OVF);
             }
           }
           rr[i] = atomicAdd(&wc[d], 1u);
@@ -960,7 +979,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       }
     }
     __syncthreads();  // LDS of bucket j free before the next commit
-    WARM_STAMP(5);
+    PIPE_STAMP(5);
     j = jn;
     b = bn;
     nseg = nseg_n;
