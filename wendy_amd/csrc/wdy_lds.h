@@ -87,12 +87,13 @@ __device__ u64 block_excl_u64(u64 v, u64 *s_red, u64 *total) {
   return wp + x - v;
 }
 
-// The same for u32 values whose total fits 32 bits.
+// The same for u32 values whose total fits 32 bits, with one barrier: the
+// caller guarantees that a barrier separates every earlier read of s_red
+// from this call.
 template <int NWAVE>
 __device__ u32 block_excl_u32(u32 v, u64 *s_red) {
   const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const u32 x = wave_incl_add(v);
-  __syncthreads();
   if (lane == 63) s_red[w] = x;
   __syncthreads();
   u32 wp = 0;
@@ -182,13 +183,20 @@ struct Lds {
 // must separate the ranking's LDS reads from later writes to s.
 // KREG: the caller's registers kr[i], ir[i] hold the key and index of its
 // sheet i * WT + t (as stored in s), so they need not be read back from LDS.
-template <class SH, bool EQ, bool KREG>
+// SCAT (with KREG; only when s.v holds nothing the caller needs, as in the
+// equal-mass warm step, which writes it after the ranking): each sheet's key
+// is also stored at its slot in s.v's space, so the in-bin comparison reads
+// the bin's keys contiguously and goes through s.ord to an index only on an
+// exact tie with another sheet.
+template <class SH, bool EQ, bool KREG, bool SCAT = false>
 __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
                                      u64 *s_mm, u64 *prof, u32 (&rk)[SH::PER],
                                      const u64 (&kr)[SH::PER],
                                      const i32 (&ir)[SH::PER]) {
   constexpr int WT = SH::WT, PER = SH::PER, NQ = SH::NQ, NWAVE = SH::NWAVE;
+  static_assert(!SCAT || KREG, "the slot-ordered keys come from registers");
   const u32 t = threadIdx.x;
+  u64 *const sk = (u64 *)s.v;  // SCAT: keys in slot order
   u64 kmin = ~0ull, kmax = 0ull;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
@@ -259,8 +267,10 @@ __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
       loc[i] = s.bins[t * WPT + i];
       sum += SH::PACK ? (loc[i] & 0xffffu) + (loc[i] >> 16) : loc[i];
     }
+    // (two barriers above separate any earlier read of s_red; the scan's
+    // barrier orders every thread's bin reads before the offsets replace
+    // them)
     u32 run = block_excl_u32<NWAVE>(sum, s_red);
-    __syncthreads();
 #pragma unroll
     for (int i = 0; i < WPT; ++i) {
       if (SH::PACK) {  // exclusive offsets of both bins of the word
@@ -274,9 +284,11 @@ __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
     }
   }
   __syncthreads();
+  u32 myslot[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const u32 e = i * WT + t;
+    myslot[i] = 0;
     if (e < c) {
       u32 slot;
       if (SH::PACK) {
@@ -286,6 +298,10 @@ __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
         slot = atomicAdd(&s.bins[q[i]], 1u);
       }
       s.ord[slot] = (unsigned short)e;
+      if (SCAT) {
+        sk[slot] = kr[i];
+        myslot[i] = slot;
+      }
     }
   }
   __syncthreads();
@@ -303,7 +319,27 @@ __device__ void lds_bucket_rank_impl(Lds<SH, EQ> &s, u32 c, u64 *s_red,
     if (e < c) {
       const u32 lo = q[i] == 0 ? 0u : bin_end(q[i] - 1), hi = bin_end(q[i]);
       u32 r = lo;
-      if (hi - lo > 1) {
+      if (SCAT && hi - lo > 1) {
+        const u64 k = kr[i];
+        const i32 id = ir[i];
+        // another sheet at slot a precedes this one if its key is smaller,
+        // or equal with a smaller index
+        auto tie = [&](u32 a) -> u32 {
+          return a != myslot[i] && s.id[s.ord[a]] < id ? 1u : 0u;
+        };
+        u32 a = lo;
+        for (; a + 2 <= hi; a += 2) {  // two at a time: the LDS loads overlap
+          const u64 k0 = sk[a], k1 = sk[a + 1];
+          r += (k0 < k ? 1u : 0u) + (k1 < k ? 1u : 0u);
+          if (k0 == k) r += tie(a);
+          if (k1 == k) r += tie(a + 1);
+        }
+        if (a < hi) {
+          const u64 k0 = sk[a];
+          r += k0 < k ? 1u : 0u;
+          if (k0 == k) r += tie(a);
+        }
+      } else if (hi - lo > 1) {
         const u64 k = KREG ? kr[i] : s.key[e];
         const i32 id = KREG ? ir[i] : s.id[e];
         u32 a = lo;
@@ -337,6 +373,15 @@ __device__ __forceinline__ void lds_bucket_rank_regs(
     Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm, u64 *prof,
     u32 (&rk)[SH::PER], const u64 (&kr)[SH::PER], const i32 (&ir)[SH::PER]) {
   lds_bucket_rank_impl<SH, EQ, true>(s, c, s_red, s_mm, prof, rk, kr, ir);
+}
+
+// The same, storing the keys in slot order in s.v's space (whose contents
+// it overwrites) for the in-bin comparison.
+template <class SH, bool EQ>
+__device__ __forceinline__ void lds_bucket_rank_regs_scat(
+    Lds<SH, EQ> &s, u32 c, u64 *s_red, u64 *s_mm, u64 *prof,
+    u32 (&rk)[SH::PER], const u64 (&kr)[SH::PER], const i32 (&ir)[SH::PER]) {
+  lds_bucket_rank_impl<SH, EQ, true, true>(s, c, s_red, s_mm, prof, rk, kr, ir);
 }
 
 // s.ord[0..c) := the permutation of 0..c-1 sorting (key, id) ascending.

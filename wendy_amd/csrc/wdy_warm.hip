@@ -705,7 +705,9 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           u64 kk = dk[i];
           if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * dv[i]);
           dk[i] = kk;  // the ranking and the force read it from here
-          s.key[e] = kk;
+          // (equal masses: the ranking stores the keys in slot order itself,
+          // and the force writes the drifted ones here)
+          if (!EQ) s.key[e] = kk;
           // equal masses: only this thread's force reads its velocities,
           // from registers, and it stores the kicked ones here itself
           if (!EQ) s.v[e] = dv[i];
@@ -747,7 +749,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
     u32 rk[PER];
     if (ok) {
       if constexpr (EQ)
-        lds_bucket_rank_regs<SH, EQ>(s, c, s_red, s_mm,
+        lds_bucket_rank_regs_scat<SH, EQ>(s, c, s_red, s_mm,
                                      prof_on ? KARG(prof) + 8ull * j : nullptr, rk,
                                      dk, di);
       else
@@ -858,10 +860,10 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           s.v[e] = v;
           s.key[e] = encode_pos(x);
           for (u32 si = sfirst; si * A.T < send; ++si)
-            if (R == si * A.T) A.wr.spl[si] = R == 0 ? 0ull : kold;
+            if (R == si * A.T) KARG(wr.spl)[si] = R == 0 ? 0ull : kold;
         }
       }
-      if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
+      if (j == nb - 1 && t == 0) KARG(wr.spl)[nb] = ~0ull;
       __syncthreads();
     } else if (write) {
       // ---- force, kick, drift in sorted order ------------------------------------
@@ -900,10 +902,10 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
             x = x + A.hd * v;
             s.v[e] = v;
             s.key[e] = encode_pos(x);
-            if (R == srank) A.wr.spl[sidx] = R == 0 ? 0ull : kold;
+            if (R == srank) KARG(wr.spl)[sidx] = R == 0 ? 0ull : kold;
           }
         }
-        if (j == nb - 1 && t == 0) A.wr.spl[nb] = ~0ull;
+        if (j == nb - 1 && t == 0) KARG(wr.spl)[nb] = ~0ull;
       }
       __syncthreads();
     }
@@ -919,6 +921,9 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
       // distinct.
       u32 *wc = s.bins;
       const u64 *spl = s_spl(b);
+      // the window's ends, read once: the compiler must assume that the LDS
+      // atomics below may write them, and would reload them for every sheet
+      const u64 spl_first = spl[0], spl_end = spl[hi - lo + 1];
       if (t <= S) wc[t] = 0;
       __syncthreads();
       u32 dd[PER], rr[PER];
@@ -932,8 +937,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           const u32 e = EQ ? r : (u32)s.ord[r];
           u64 kp = s.key[e];
           if (A.hp != 0.0) kp = encode_pos(decode_pos(kp) + A.hp * s.v[e]);
-          const bool below = lo > 0 && kp < spl[0];
-          const bool above = hi + 1 < nb && kp >= spl[hi - lo + 1];
+          const bool below = lo > 0 && kp < spl_first;
+          const bool above = hi + 1 < nb && kp >= spl_end;
           if (!below && !above) {
             // the largest a with spl[a] <= kp: spl[0] <= kp < spl[hi-lo+1],
             // and the window is padded past hi-lo+1
