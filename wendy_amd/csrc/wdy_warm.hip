@@ -840,6 +840,10 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         publish(A.status + STATUS_WORDS * jn, ep, jn == 0 ? 2u : 1u, c_n, 0, false);
     }
 
+    // equal masses: the drifted keys and velocities stay in the registers of the thread
+    // that owns the sheet until the write
+    u64 kn[PER];
+    double vn[PER];
     if (write && EQ) {
       // ---- force, kick, drift of each thread's own sheets, by rank ------------------
       // (the look-back's barrier separates the ranking's reads from these
@@ -857,14 +861,14 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           const double a = (EXT ? ext_accel(A.ext, x, A.t) : 0.0) + F;
           const double v = dv[i] + A.h * a;
           x = x + A.hd * v;
-          s.v[e] = v;
-          s.key[e] = encode_pos(x);
+          vn[i] = v;
+          kn[i] = encode_pos(x);
           for (u32 si = sfirst; si * A.T < send; ++si)
             if (R == si * A.T) KARG(wr.spl)[si] = R == 0 ? 0ull : kold;
         }
       }
       if (j == nb - 1 && t == 0) KARG(wr.spl)[nb] = ~0ull;
-      __syncthreads();
+      // (no barrier: the partition reads only this thread's sheets)
     } else if (write) {
       // ---- force, kick, drift in sorted order ------------------------------------
       {
@@ -935,8 +939,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         rr[i] = 0;
         if (valid) {
           const u32 e = EQ ? r : (u32)s.ord[r];
-          u64 kp = s.key[e];
-          if (A.hp != 0.0) kp = encode_pos(decode_pos(kp) + A.hp * s.v[e]);
+          u64 kp = EQ ? kn[i] : s.key[e];
+          if (A.hp != 0.0) kp = encode_pos(decode_pos(kp) + A.hp * (EQ ? vn[i] : s.v[e]));
           const bool below = lo > 0 && kp < spl_first;
           const bool above = hi + 1 < nb && kp >= spl_end;
           if (!below && !above) {
@@ -948,8 +952,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
             const u32 dst = bucket_of(A.rd.spl, nb, kp);
             const u32 q = atomicAdd(KARG(ovf_w), 1u);
             if (q < WARM_OVF_CAP) {
-              KARG(wr.okey)[q] = s.key[e];
-              KARG(wr.ov)[q] = s.v[e];
+              KARG(wr.okey)[q] = EQ ? kn[i] : s.key[e];
+              KARG(wr.ov)[q] = EQ ? vn[i] : s.v[e];
               KARG(wr.oid)[q] = s.id[e];
               KARG(wr.om)[q] = EQ ? A.mq : s.m[e];
               KARG(wr.odst)[q] = dst;
@@ -976,8 +980,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         if (r < c) {
           const u32 e = EQ ? r : (u32)s.ord[r];
           const u32 p = off + wc[dd[i]] + rr[i];
-          A.kout[p] = s.key[e];
-          A.vout[p] = s.v[e];
+          A.kout[p] = EQ ? kn[i] : s.key[e];
+          A.vout[p] = EQ ? vn[i] : s.v[e];
           A.iout[p] = s.id[e];
           if (!EQ) A.mout[p] = s.m[e];
         }
