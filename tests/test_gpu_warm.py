@@ -246,7 +246,8 @@ def test_window_fitting_ladder(gpu_lib, h, shape):
 
 @pytest.mark.parametrize('lg,mass,shape', [
     (14, 'equal', 3), (16, 'lognormal', 3), (18, 'equal', 1),
-    (20, 'equal', 5), (20, 'lognormal', 6), (21, 'lognormal', 6)])
+    (20, 'equal', 5), (20, 'lognormal', 6), (21, 'lognormal', 6),
+    (21, 'equal', 0), (22, 'lognormal', 7)])
 def test_initial_shape_by_size(gpu_lib, lg, mass, shape):
     """A simulation that has the GPU to itself starts from the tile shape
     measured fastest for its size (larger workgroups while its buckets do not

@@ -42,8 +42,11 @@ const WarmShape &warm_shape(int v);
 // 2^14), 2^17-2^18 shape 1 (512 threads, 2048 sheets; 0.57 -> 0.51 at
 // 2^18), 2^19-2^20 shape 5 (512 threads, 3072 sheets) with equal masses and
 // shape 6 (768 threads, 3072 sheets) with unequal ones (1.46 -> 0.95 equal,
-// 1.57 -> 1.09 lognormal at 2^20), 2^21 shape 6 (within 3% of the best),
-// from 2^22 the default (every shape within 2%).
+// 1.57 -> 1.09 lognormal at 2^20); 2^21-2^22 shape 0 (1024 threads, 4096
+// sheets) with equal masses (2^21: 1.40 with shape 6 -> 1.21; 2^22: 2.36 ->
+// 1.93), 2^21 shape 6 and 2^22-2^23 shape 7 (256 threads, 2048 sheets) with
+// unequal ones (2^22: 3.56 -> 2.67), 2^23 shape 7 with equal ones (3.67 ->
+// 3.34), from 2^24 the default.
 int warm_initial_shape(int64_t n, bool equal_mass);
 inline u32 warm_default_target(int cap) {
   const u32 c = (u32)cap, room = c / 5u > 256u ? c / 5u : 256u;

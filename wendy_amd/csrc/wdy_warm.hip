@@ -1150,7 +1150,9 @@ int warm_initial_shape(int64_t n, bool equal_mass) {
   if (lg < 16.5) return 3;
   if (lg < 18.5) return 1;
   if (lg < 20.5) return equal_mass ? 5 : 6;
-  if (lg < 21.5) return 6;
+  if (lg < 21.5) return equal_mass ? 0 : 6;
+  if (lg < 22.5) return equal_mass ? 0 : WARM_SHAPE_WIDE;
+  if (lg < 23.5) return WARM_SHAPE_WIDE;
   return WARM_SHAPE_DEFAULT;
 }
 
