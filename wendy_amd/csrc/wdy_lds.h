@@ -184,7 +184,7 @@ struct Lds {
 // KREG: the caller's registers kr[i], ir[i] hold the key and index of its
 // sheet i * WT + t (as stored in s), so they need not be read back from LDS.
 // SCAT (with KREG; only when s.v holds nothing the caller needs, as in the
-// equal-mass warm step, which writes it after the ranking): each sheet's key
+// equal-mass warm step, whose velocities stay in registers): each sheet's key
 // is also stored at its slot in s.v's space, so the in-bin comparison reads
 // the bin's keys contiguously and goes through s.ord to an index only on an
 // exact tie with another sheet.

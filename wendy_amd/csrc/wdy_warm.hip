@@ -706,7 +706,7 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
           if (A.pre != 0.0) kk = encode_pos(decode_pos(kk) + A.pre * dv[i]);
           dk[i] = kk;  // the ranking and the force read it from here
           // (equal masses: the ranking stores the keys in slot order itself,
-          // and the force writes the drifted ones here)
+          // and the drifted ones stay in the owning thread's registers)
           if (!EQ) s.key[e] = kk;
           // equal masses: only this thread's force reads its velocities,
           // from registers, and it stores the kicked ones here itself
@@ -840,8 +840,8 @@ __attribute__((amdgpu_waves_per_eu(EQ ? SH::OCC_EQ : 1))) void k_warm_pipe(WarmA
         publish(A.status + STATUS_WORDS * jn, ep, jn == 0 ? 2u : 1u, c_n, 0, false);
     }
 
-    // equal masses: the drifted keys and velocities stay in the registers of the thread
-    // that owns the sheet until the write
+    // equal masses: the drifted keys and velocities stay in the registers
+    // of the thread that owns the sheet until the write
     u64 kn[PER];
     double vn[PER];
     if (write && EQ) {
