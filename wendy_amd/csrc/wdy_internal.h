@@ -131,6 +131,7 @@ int timer_collect(Timer *t, int kc, double *ms_total, int64_t *count);
 // one all-digit histogram pass plus one onesweep scatter pass per digit.
 // --------------------------------------------------------------------------
 constexpr u32 RADIX_SPIN_LIMIT = 1u << 20;  // probes of one look-back wait
+constexpr u32 RADIX_TILE_KEYS = 4096;       // keys one scatter workgroup ranks
 struct RadixWork {
   u32 n = 0;
   u32 ntiles = 0;

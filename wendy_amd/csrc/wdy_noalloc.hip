@@ -20,8 +20,8 @@ namespace wdy {
 namespace {
 
 constexpr u32 MAX_N = WDY_NOALLOC_MAX_N;
-constexpr size_t STATUS_BYTES =
-    ((size_t)(MAX_N + 4095) / 4096) * 256 * sizeof(u64);  // 2 KiB per tile
+constexpr size_t STATUS_BYTES =  // 2 KiB per radix tile
+    ((size_t)(MAX_N + RADIX_TILE_KEYS - 1) / RADIX_TILE_KEYS) * 256 * sizeof(u64);
 constexpr size_t WORK_BYTES = (size_t)49 << 20;
 constexpr size_t SCRATCH_BYTES = STATUS_BYTES + WORK_BYTES;
 

@@ -26,8 +26,9 @@ namespace {
 constexpr int RADIX = 256;
 constexpr int THREADS = 256;
 constexpr int WAVES = THREADS / 64;
-constexpr int ITEMS = 16;
-constexpr int TILE = THREADS * ITEMS;  // 4096 keys per workgroup
+constexpr int ITEMS = (int)RADIX_TILE_KEYS / THREADS;
+constexpr int TILE = THREADS * ITEMS;  // RADIX_TILE_KEYS keys per workgroup
+static_assert(TILE == (int)RADIX_TILE_KEYS, "tile");
 constexpr int PASSES = 8;
 
 __device__ __forceinline__ u64 atomic_load_agent(const u64 *p) {
