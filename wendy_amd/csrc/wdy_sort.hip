@@ -6,7 +6,7 @@
 //     (8 x 256 bins) -- the "one key-read histogram pass" of a onesweep sort;
 //   * each of the 8 passes is ONE scatter kernel.  A workgroup takes a ticket
 //     (tile id in launch order, so every earlier tile is already resident),
-//     ranks its 4096 keys stably with wave64 ballots (8 ballots give each lane
+//     ranks its 6144 keys stably with wave64 ballots (8 ballots give each lane
 //     the mask of lanes holding the same digit; popcount below the lane is its
 //     rank in the row), publishes its per-digit counts and resolves its global
 //     per-digit offsets with a decoupled look-back over earlier tiles, stages
@@ -110,8 +110,9 @@ __global__ __launch_bounds__(THREADS) void k_scatter(
     u64 *__restrict__ kout, u32 *__restrict__ pout, u32 n, u32 shift,
     const u32 *__restrict__ offs, u64 *status, u32 *counter, u32 epoch,
     u32 spin_limit, u32 *err) {
-  // 54,272 bytes (106 allocation blocks of 512): three workgroups fit a
-  // CU's 160 KiB.  The tile number and the scan's wave totals live in the
+  // 78,848 bytes: two workgroups fit a CU's 160 KiB (the 212 VGPRs of 24
+  // keys per thread allow two waves per SIMD as well).  The tile number
+  // and the scan's wave totals live in the
   // key staging area, which is written only after both are consumed; the
   // digits' global bases replace the tile-local ones once the local
   // scatter has read those.
