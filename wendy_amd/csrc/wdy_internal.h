@@ -134,7 +134,7 @@ constexpr u32 RADIX_SPIN_LIMIT = 1u << 20;  // probes of one look-back wait
 // keys one scatter workgroup ranks: 24 per thread.  Measured per scatter
 // pass at N=2^24: 2048 keys 216.8 us, 3072 188.3, 4096 173.5, 5120 158.1,
 // 6144 147.5 (fewer tiles, fewer look-back links; 6144 is the largest whose
-// 78,848 B of LDS still fit two workgroups in a CU)
+// 78,848 B of LDS still fit two workgroups in a CU), 8192 (one per CU) 182.9
 constexpr u32 RADIX_TILE_KEYS = 6144;
 struct RadixWork {
   u32 n = 0;
