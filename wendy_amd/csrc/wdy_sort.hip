@@ -66,7 +66,11 @@ __global__ void k_sort_init(u32 *hist, u32 *counters, u32 clear_err) {
   if (clear_err && threadIdx.x == 0) counters[ERR_WORD] = 0;
 }
 
-// Counts all 8 digits of every key.  Grid-stride over tiles.
+// Counts all 8 digits of every key.  Grid-stride over tiles.  The high
+// digits of positions that share a sign and exponent range take few values,
+// so most lanes of a wave would add to the same LDS word; the lanes whose
+// digit equals the first active lane's are counted by that lane with one
+// atomic, the others add one each.
 __global__ __launch_bounds__(THREADS) void k_hist(const u64 *__restrict__ keys,
                                                   u32 n, u32 ntiles,
                                                   u32 *__restrict__ hist) {
@@ -74,6 +78,7 @@ __global__ __launch_bounds__(THREADS) void k_hist(const u64 *__restrict__ keys,
   for (u32 i = threadIdx.x; i < PASSES * RADIX; i += THREADS)
     (&s_h[0][0])[i] = 0;
   __syncthreads();
+  const u32 lane = threadIdx.x & 63;
   for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const u32 base = tile * TILE;
 #pragma unroll 4
@@ -81,9 +86,16 @@ __global__ __launch_bounds__(THREADS) void k_hist(const u64 *__restrict__ keys,
       const u32 i = base + j * THREADS + threadIdx.x;
       if (i < n) {
         const u64 k = keys[i];
+        const u64 act = __ballot(1);
+        const u32 first = (u32)__ffsll((long long)act) - 1u;
 #pragma unroll
-        for (int p = 0; p < PASSES; ++p)
-          atomicAdd(&s_h[p][(k >> (8 * p)) & 255], 1u);
+        for (int p = 0; p < PASSES; ++p) {
+          const u32 d = (u32)(k >> (8 * p)) & 255u;
+          const u32 d0 = __builtin_amdgcn_readfirstlane(d);
+          const u64 same = __ballot(d == d0);
+          if (lane == first) atomicAdd(&s_h[p][d0], (u32)__popcll(same));
+          if (d != d0) atomicAdd(&s_h[p][d], 1u);
+        }
       }
     }
   }
